@@ -1,0 +1,216 @@
+/*
+ * octofmm.h -- C-ABI of the MI355X (gfx950) FMM gravity interaction kernels.
+ *
+ * This is the drop-in boundary for Octo-Tiger's cell-to-cell gravity
+ * interaction path.  Every entry point takes plain pointers and sizes; no
+ * C++ or PyTorch types cross it.  The reference functions each entry point
+ * replaces are cited next to it (paths relative to the Octo-Tiger source
+ * tree).
+ *
+ * Conventions (shared by every entry point):
+ *   - A sub-grid is 8x8x8 cells.  A cell's flat index is x*64 + y*8 + z.
+ *   - Multipoles / expansions are 20 doubles per cell in Octo-Tiger's
+ *     taylor<4> component order: [0] mass, [1..3] x y z, [4..9] xx xy xz yy yz
+ *     zz, [10..19] xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz
+ *     (octotiger/taylor.hpp:346-420, src/taylor.cpp:10-58).
+ *   - Neighbour directions use geo::direction's flat order: raw index
+ *     i = (dx+1) + 3*(dy+1) + 9*(dz+1) in 0..26, the centre (13) skipped, so
+ *     flat index = i < 13 ? i : i - 1 (octotiger/geometry.hpp:91-194).
+ *   - L is accumulated (+=), L_c is overwritten, exactly as the reference
+ *     writes grid::get_L() and grid::get_L_c()
+ *     (octotiger/multipole_interactions/kernel/kokkos_kernel.hpp:966-981).
+ *   - Errors are reported as octofmm_status codes; nothing aborts.
+ */
+#ifndef OCTOFMM_H
+#define OCTOFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCTOFMM_INX 8
+#define OCTOFMM_CELLS 512
+#define OCTOFMM_NCOMP 20
+#define OCTOFMM_NDIR 26
+#define OCTOFMM_ABI_VERSION 1
+
+typedef enum octofmm_status {
+    OCTOFMM_OK = 0,
+    OCTOFMM_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad count, bad kind */
+    OCTOFMM_ERR_THETA = 2,            /* theta outside [theta_min, 1/2] or stencil check failed */
+    OCTOFMM_ERR_NO_DEVICE = 3,        /* no HIP device / bad ordinal */
+    OCTOFMM_ERR_HIP = 4,              /* a HIP runtime call failed */
+    OCTOFMM_ERR_ALLOCATION = 5,       /* device or pinned host allocation failed */
+    OCTOFMM_ERR_SIZE = 6,             /* a size exceeds what the plan / library supports */
+    OCTOFMM_ERR_NONFINITE = 7         /* non-finite input detected by a checking entry point */
+} octofmm_status;
+
+/* gsolve_type (octotiger/defs.hpp:73-75) */
+typedef enum octofmm_solve_type { OCTOFMM_RHO = 0, OCTOFMM_DRHODT = 1 } octofmm_solve_type;
+
+/* What a neighbour direction holds (neighbor_gravity_type,
+ * octotiger/interaction_types.hpp:27-64): a refined (non-leaf) neighbour hands
+ * over multipoles and centres of mass, a leaf neighbour hands over monopoles,
+ * an empty direction (physical boundary) contributes zeros. */
+typedef enum octofmm_neighbor_kind {
+    OCTOFMM_NEIGHBOR_EMPTY = 0,
+    OCTOFMM_NEIGHBOR_MULTIPOLE = 1,
+    OCTOFMM_NEIGHBOR_MONOPOLE = 2
+} octofmm_neighbor_kind;
+
+typedef enum octofmm_precision {
+    OCTOFMM_PRECISION_FP64 = 0,
+    /* FP32 pair arithmetic for far offsets with |d|^2 > mixed_far_d2 (cell
+     * units); FP64 accumulation, FP64 angular-momentum correction, FP64 near
+     * offsets, root kernel always FP64. */
+    OCTOFMM_PRECISION_MIXED = 1
+} octofmm_precision;
+
+typedef struct octofmm_neighbor {
+    int32_t kind;          /* octofmm_neighbor_kind */
+    int32_t count;         /* cells provided: 512 = full sub-grid, fewer = partial halo */
+    const double* M;       /* [count][20] multipoles (MULTIPOLE), else NULL */
+    const double* com;     /* [count][3]  centres of mass (MULTIPOLE), else NULL */
+    const double* mon;     /* [count]     monopoles (MONOPOLE), else NULL */
+    const int32_t* cells;  /* [count] neighbour-local flat indices of a partial halo, in the
+                              order the data arrays hold them (grid::get_ilist_n_bnd order);
+                              NULL when count == 512 */
+} octofmm_neighbor;
+
+/* One sub-grid's inputs, as multipole_interaction_interface::update_input and
+ * monopole_interactions::update_input receive them
+ * (octotiger/multipole_interactions/legacy/multipole_interaction_interface.hpp:88-96,
+ *  octotiger/monopole_interactions/legacy/monopole_interaction_interface.hpp:64-68). */
+typedef struct octofmm_subgrid {
+    const double* M;       /* [512][20] own multipoles (non-leaf targets and root) */
+    const double* com;     /* [512][3]  own centres of mass, com_ptr[0] */
+    const double* mon;     /* [512]     own monopoles (leaf targets) */
+    octofmm_neighbor neighbors[OCTOFMM_NDIR];
+    double dx;             /* cell width */
+    double xbase[3];       /* centre of own cell (0,0,0) */
+    int32_t type;          /* octofmm_solve_type */
+    int32_t use_root_stencil;
+} octofmm_subgrid;
+
+typedef struct octofmm_config {
+    double theta;          /* opening criterion, Octo-Tiger default 0.5 */
+    int32_t device;        /* HIP device ordinal */
+    int32_t reserved;
+} octofmm_config;
+
+typedef struct octofmm_context octofmm_context;
+
+/* Creates a context on one GPU and builds the stencils by the reference's
+ * rule (src/multipole_interactions/util/calculate_stencil.cpp:23-161,
+ * src/monopole_interactions/util/calculate_stencil.cpp:19-177).  Replaces the
+ * start-up stencil initialisation of frontend/init_methods.cpp:189-192 and
+ * init_fmm_constants (src/cuda_util/cuda_scheduler.cpp:31-90).  A context may be
+ * used by one host thread at a time. */
+octofmm_status octofmm_create(const octofmm_config* config, octofmm_context** ctx);
+octofmm_status octofmm_destroy(octofmm_context* ctx);
+const char* octofmm_status_string(octofmm_status status);
+/* Text of the last error recorded on the context (never NULL). */
+const char* octofmm_last_error(const octofmm_context* ctx);
+int32_t octofmm_abi_version(void);
+
+/* Drop-in for
+ *   multipole_interaction_interface::compute_multipole_interactions(monopoles, M_ptr, com_ptr,
+ *       neighbors, type, dx, is_direction_empty, xbase, use_root_stencil)
+ *   (octotiger/multipole_interactions/legacy/multipole_interaction_interface.hpp:38-43,
+ *    src/multipole_interactions/legacy/multipole_interaction_interface.cpp:82-179,
+ *    its CUDA twin src/multipole_interactions/legacy/cuda_multipole_interaction_interface.cpp:56-279,
+ *    and the free function multipole_kernel_interface,
+ *    src/multipole_interactions/multipole_kernel_interface.cpp:73-169).
+ * Non-leaf target: M2L over the theta stencil (non-root) or over all pairs of
+ * the root sub-grid (use_root_stencil).  Host pointers in, synchronous.
+ *   L   [512][20]  accumulated
+ *   L_c [512][3]   overwritten (RHO only; untouched for DRHODT) */
+octofmm_status octofmm_compute_multipole_interactions(octofmm_context* ctx,
+    const octofmm_subgrid* subgrid, double* L, double* L_c);
+
+/* Drop-in for
+ *   monopole_interaction_interface::compute_interactions(monopoles, com_ptr, neighbors, type, dx,
+ *       is_direction_empty, grid, contains_multipole_neighbor)
+ *   (octotiger/monopole_interactions/legacy/monopole_interaction_interface.hpp:35-39,
+ *    src/monopole_interactions/legacy/monopole_interaction_interface.cpp,
+ *    and monopole_kernel_interface, src/monopole_interactions/monopole_kernel_interface.cpp:97-200).
+ * Leaf target: p2p against own and leaf-neighbour monopoles, plus p2m against
+ * every refined (MULTIPOLE) neighbour when contains_multipole_neighbor != 0.
+ *   L   [512][20]  components 0..3 accumulated
+ *   L_c [512][3]   overwritten when RHO and contains_multipole_neighbor, else untouched */
+octofmm_status octofmm_compute_monopole_interactions(octofmm_context* ctx,
+    const octofmm_subgrid* subgrid, int32_t contains_multipole_neighbor, double* L, double* L_c);
+
+/* ---------------------------------------------------------------------------
+ * Level batch: every sub-grid of one octree level in one device pass.
+ * This has no single counterpart in the reference; it batches what
+ * node_server::compute_fmm (src/node_server.cpp:589-798) issues as one call of
+ * the two entry points above per octree node, and generalises the reference's
+ * aggregated p2p launch (octotiger/monopole_interactions/kernel/kokkos_kernel.hpp:321-514).
+ * All data stays on the device.  A plan is built once per level shape and
+ * executed every solve.
+ * ------------------------------------------------------------------------- */
+typedef enum octofmm_slot_kind {
+    OCTOFMM_SLOT_NONLEAF = 0,  /* refined: holds multipoles (M, com) */
+    OCTOFMM_SLOT_LEAF = 1      /* leaf: holds monopoles (mon); seen by neighbours at cell centres */
+} octofmm_slot_kind;
+
+typedef struct octofmm_level {
+    int32_t nslots;            /* sub-grid slots in the device arrays */
+    const int32_t* neighbors;  /* host [nslots][27] slot of the neighbour in raw direction
+                                  (dx+1)+3(dy+1)+9(dz+1); entry 13 is the slot itself;
+                                  -1 = empty direction */
+    const int32_t* kind;       /* host [nslots] octofmm_slot_kind */
+    const double* xbase;       /* host [nslots][3] centre of each slot's cell (0,0,0) */
+    double dx;
+    int32_t ntargets;
+    const int32_t* targets;    /* host [ntargets] slots whose interactions are computed */
+    int32_t type;              /* octofmm_solve_type */
+    int32_t precision;         /* octofmm_precision */
+    int32_t mixed_far_d2;      /* MIXED: FP32 for far offsets with |d|^2 > mixed_far_d2 (4 = "beyond 2 cells") */
+    int32_t flags;             /* OCTOFMM_PLAN_* */
+    /* device arrays, AoS per slot, caller-owned */
+    const double* d_M;         /* [nslots][512][20] multipoles (zeros in leaf slots) */
+    const double* d_com;       /* [nslots][512][3]  centres of mass */
+    const double* d_mon;       /* [nslots][512]     monopoles (zeros in non-leaf slots) */
+    double* d_L;               /* [nslots][512][20] accumulated for target slots */
+    double* d_L_c;             /* [nslots][512][3]  overwritten for target slots (RHO) */
+} octofmm_level;
+
+#define OCTOFMM_PLAN_USE_GRAPH 1   /* capture the level pass in one hipGraph */
+#define OCTOFMM_PLAN_AUTOTUNE 2    /* time the kernel variants once and keep the fastest */
+
+typedef struct octofmm_plan octofmm_plan;
+
+typedef struct octofmm_plan_stats {
+    int64_t far_pairs;         /* M2L pairs with far (multipole) partners */
+    int64_t near_pairs;        /* M2L near pairs with monopole partners (evaluated) */
+    int64_t p2p_pairs;         /* leaf p2p pairs */
+    int64_t p2m_pairs;         /* leaf p2m pairs */
+    int64_t flops;             /* nominal FLOPs of one execution */
+    int64_t mixed_far_pairs;   /* far pairs evaluated in FP32 (MIXED) */
+    int32_t n_m2l_items;
+    int32_t n_leaf_items;
+    int32_t variant;           /* kernel variant chosen */
+    int32_t reserved;
+} octofmm_plan_stats;
+
+octofmm_status octofmm_level_plan_create(octofmm_context* ctx, const octofmm_level* level,
+    octofmm_plan** plan);
+/* Executes the plan on a HIP stream (hipStream_t, NULL = default stream); asynchronous. */
+octofmm_status octofmm_level_plan_execute(octofmm_plan* plan, void* stream);
+octofmm_status octofmm_level_plan_stats(const octofmm_plan* plan, octofmm_plan_stats* stats);
+/* Runs each stage of the plan `iters` times on `stream`, bracketed by HIP
+ * events recorded on that stream, and writes the average milliseconds per
+ * stage: ms[0] AoS->SoA staging, ms[1] interaction kernel, ms[2] write-back. */
+octofmm_status octofmm_level_plan_time_stages(octofmm_plan* plan, void* stream, int32_t iters,
+    double* ms);
+octofmm_status octofmm_level_plan_destroy(octofmm_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCTOFMM_H */

@@ -1,0 +1,139 @@
+"""ctypes mirror of include/octofmm.h (struct layouts and enums).
+
+Shared by the product binding (octotiger_amd.native) and by the test-side
+oracle binding, so both pass identical input descriptors across the C-ABI.
+"""
+import ctypes
+
+INX = 8
+CELLS = 512
+NCOMP = 20
+NDIR = 26
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_THETA = 2
+ERR_NO_DEVICE = 3
+ERR_HIP = 4
+ERR_ALLOCATION = 5
+ERR_SIZE = 6
+ERR_NONFINITE = 7
+
+RHO = 0
+DRHODT = 1
+
+NEIGHBOR_EMPTY = 0
+NEIGHBOR_MULTIPOLE = 1
+NEIGHBOR_MONOPOLE = 2
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+SLOT_NONLEAF = 0
+SLOT_LEAF = 1
+
+PLAN_USE_GRAPH = 1
+PLAN_AUTOTUNE = 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class Neighbor(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("count", ctypes.c_int32),
+        ("M", _dp),
+        ("com", _dp),
+        ("mon", _dp),
+        ("cells", _ip),
+    ]
+
+
+class Subgrid(ctypes.Structure):
+    _fields_ = [
+        ("M", _dp),
+        ("com", _dp),
+        ("mon", _dp),
+        ("neighbors", Neighbor * NDIR),
+        ("dx", ctypes.c_double),
+        ("xbase", ctypes.c_double * 3),
+        ("type", ctypes.c_int32),
+        ("use_root_stencil", ctypes.c_int32),
+    ]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("theta", ctypes.c_double),
+        ("device", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class Level(ctypes.Structure):
+    _fields_ = [
+        ("nslots", ctypes.c_int32),
+        ("neighbors", _ip),
+        ("kind", _ip),
+        ("xbase", _dp),
+        ("dx", ctypes.c_double),
+        ("ntargets", ctypes.c_int32),
+        ("targets", _ip),
+        ("type", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("mixed_far_d2", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("d_M", ctypes.c_void_p),
+        ("d_com", ctypes.c_void_p),
+        ("d_mon", ctypes.c_void_p),
+        ("d_L", ctypes.c_void_p),
+        ("d_L_c", ctypes.c_void_p),
+    ]
+
+
+class PlanStats(ctypes.Structure):
+    _fields_ = [
+        ("far_pairs", ctypes.c_int64),
+        ("near_pairs", ctypes.c_int64),
+        ("p2p_pairs", ctypes.c_int64),
+        ("p2m_pairs", ctypes.c_int64),
+        ("flops", ctypes.c_int64),
+        ("mixed_far_pairs", ctypes.c_int64),
+        ("n_m2l_items", ctypes.c_int32),
+        ("n_leaf_items", ctypes.c_int32),
+        ("variant", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 numpy array (or NULL for None)."""
+    if a is None:
+        return None
+    assert a.dtype.name == "float64" and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    assert a.dtype.name == "int32" and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def raw_to_flat(raw):
+    """geo::direction raw index (0..26, centre 13) -> flat neighbour index."""
+    return raw if raw < 13 else raw - 1
+
+
+def flat_to_raw(flat):
+    return flat if flat < 13 else flat + 1
+
+
+def dir_components(raw):
+    return (raw % 3 - 1, (raw // 3) % 3 - 1, raw // 9 - 1)
+
+
+def raw_index(dx, dy, dz):
+    return (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)

@@ -1,0 +1,280 @@
+"""Deterministic synthetic gravity states for the interaction kernels.
+
+Every workload is a *level*: a block of 8x8x8-cell sub-grid slots with a
+27-entry neighbour table per slot (raw geo::direction order, centre = the
+slot itself, -1 = empty direction), a slot kind (non-leaf: multipoles, leaf:
+monopoles) and the per-cell state in Octo-Tiger's AoS layout:
+
+    M   [nslots, 512, 20]  multipoles (taylor<4> order), zeros in leaf slots
+    com [nslots, 512, 3]   centres of mass (geometric cell centres in leaves)
+    mon [nslots, 512]      monopoles, zeros in non-leaf slots
+
+Random numbers come from SplitMix64 counters turned into uniforms on [0, 1)
+and Box-Muller normals, so any config is reproducible from its seed alone.
+Multipoles are built from 8 point masses per cell at the child-cell centres
+(x +- dx/4): mass = sum, centre of mass = mass-weighted mean, and the order
+1..3 moments sum m (x - com)^alpha, which is what the M2M shift operator
+taylor::operator>>= (octotiger/taylor.hpp:222-255) produces for point masses.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+
+SEED_BASE = 20261202
+
+# order-1..3 multi-indices in taylor<4> component order (octotiger/taylor.hpp:346-420)
+_ALPHA = [
+    (1, 0, 0), (0, 1, 0), (0, 0, 1),
+    (2, 0, 0), (1, 1, 0), (1, 0, 1), (0, 2, 0), (0, 1, 1), (0, 0, 2),
+    (3, 0, 0), (2, 1, 0), (2, 0, 1), (1, 2, 0), (1, 1, 1), (1, 0, 2),
+    (0, 3, 0), (0, 2, 1), (0, 1, 2), (0, 0, 3),
+]
+
+_U64 = np.uint64
+
+
+def splitmix64(seed, counters):
+    """SplitMix64 output for the given uint64 counters (vectorised)."""
+    with np.errstate(over="ignore"):
+        z = _U64(seed) + (counters.astype(np.uint64) + _U64(1)) * _U64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> _U64(30))) * _U64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> _U64(27))) * _U64(0x94D049BB133111EB)
+        return z ^ (z >> _U64(31))
+
+
+def uniform(seed, counters):
+    return (splitmix64(seed, counters) >> _U64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed, counters):
+    """Box-Muller standard normal from counters 2k and 2k+1."""
+    c = counters.astype(np.uint64) * _U64(2)
+    u1 = 1.0 - uniform(seed, c)
+    u2 = uniform(seed, c + _U64(1))
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+@dataclass
+class Level:
+    name: str
+    dx: float
+    shape: tuple                 # slots per dimension
+    neighbors: np.ndarray        # int32 [nslots, 27]
+    kind: np.ndarray             # int32 [nslots]
+    xbase: np.ndarray            # float64 [nslots, 3]
+    targets: np.ndarray          # int32 [ntargets]
+    M: np.ndarray                # float64 [nslots, 512, 20]
+    com: np.ndarray              # float64 [nslots, 512, 3]
+    mon: np.ndarray              # float64 [nslots, 512]
+    info: dict = field(default_factory=dict)
+
+    @property
+    def nslots(self):
+        return self.kind.shape[0]
+
+
+def _lattice_moments(seed, shape_cells, dx, origin, density=None, noise_sigma=1.0):
+    """Per-cell mass, centre of mass and 20-component multipole from 8 child
+    point masses.  Child mass = (dx/2)^3 * rho(child) * LogNormal(0, sigma)."""
+    nx, ny, nz = shape_cells
+    ncell = nx * ny * nz
+    gi = np.arange(ncell, dtype=np.int64)
+    ix, iy, iz = gi // (ny * nz), (gi // nz) % ny, gi % nz
+    centre = np.stack([origin[0] + (ix + 0.5) * dx, origin[1] + (iy + 0.5) * dx,
+                       origin[2] + (iz + 0.5) * dx], axis=1)
+    off = np.array([[(c >> 2) & 1, (c >> 1) & 1, c & 1] for c in range(8)], dtype=np.float64)
+    off = (off - 0.5) * (0.5 * dx)                           # +- dx/4
+    pos = centre[:, None, :] + off[None, :, :]               # [ncell, 8, 3]
+    counters = (gi[:, None] * 8 + np.arange(8)[None, :]).astype(np.uint64)
+    m = np.exp(noise_sigma * normal(seed, counters)) * (0.5 * dx) ** 3
+    if density is not None:
+        m = m * density(pos)
+    mass = m.sum(axis=1)
+    com = (m[:, :, None] * pos).sum(axis=1) / mass[:, None]
+    rel = pos - com[:, None, :]
+    M = np.empty((ncell, 20))
+    M[:, 0] = mass
+    for k, (a, b, c) in enumerate(_ALPHA):
+        M[:, k + 1] = (m * rel[:, :, 0] ** a * rel[:, :, 1] ** b * rel[:, :, 2] ** c).sum(axis=1)
+    return centre, mass, com, M
+
+
+def _to_slots(arr, shape_slots, comps):
+    """[NX*NY*NZ cells, comps] (x-major lattice) -> [nslots, 512, comps]."""
+    sx, sy, sz = shape_slots
+    a = arr.reshape(sx, 8, sy, 8, sz, 8, comps)
+    a = a.transpose(0, 2, 4, 1, 3, 5, 6)
+    return np.ascontiguousarray(a.reshape(sx * sy * sz, 512, comps))
+
+
+def _neighbor_table(shape_slots, present=None):
+    sx, sy, sz = shape_slots
+    n = sx * sy * sz
+    nb = np.full((n, 27), -1, dtype=np.int32)
+    for s in range(n):
+        a, b, c = s // (sy * sz), (s // sz) % sy, s % sz
+        for raw in range(27):
+            dxs, dys, dzs = _abi.dir_components(raw)
+            aa, bb, cc = a + dxs, b + dys, c + dzs
+            if 0 <= aa < sx and 0 <= bb < sy and 0 <= cc < sz:
+                t = (aa * sy + bb) * sz + cc
+                if present is None or present[t]:
+                    nb[s, raw] = t
+    return nb
+
+
+def _slot_xbase(shape_slots, dx, origin):
+    sx, sy, sz = shape_slots
+    s = np.arange(sx * sy * sz)
+    a, b, c = s // (sy * sz), (s // sz) % sy, s % sz
+    return np.stack([origin[0] + (8 * a + 0.5) * dx, origin[1] + (8 * b + 0.5) * dx,
+                     origin[2] + (8 * c + 0.5) * dx], axis=1)
+
+
+def uniform_level(shape_slots, target_box, seed, dx=None, name="uniform"):
+    """Non-leaf slots on a full block; targets = slots inside target_box
+    ((lo, hi) per dimension, half open)."""
+    sx, sy, sz = shape_slots
+    if dx is None:
+        dx = 1.0 / (8 * max(shape_slots))
+    origin = (0.0, 0.0, 0.0)
+    _, _, com, M = _lattice_moments(seed, (8 * sx, 8 * sy, 8 * sz), dx, origin)
+    Ms = _to_slots(M, shape_slots, 20)
+    coms = _to_slots(com, shape_slots, 3)
+    nsl = sx * sy * sz
+    kind = np.full(nsl, _abi.SLOT_NONLEAF, dtype=np.int32)
+    mon = np.zeros((nsl, 512))
+    tg = []
+    for s in range(nsl):
+        a, b, c = s // (sy * sz), (s // sz) % sy, s % sz
+        if all(lo <= v < hi for v, (lo, hi) in zip((a, b, c), target_box)):
+            tg.append(s)
+    return Level(name=name, dx=dx, shape=tuple(shape_slots), neighbors=_neighbor_table(shape_slots),
+                 kind=kind, xbase=_slot_xbase(shape_slots, dx, origin),
+                 targets=np.array(tg, dtype=np.int32), M=Ms, com=coms, mon=mon,
+                 info={"seed": seed})
+
+
+def config1(seed=SEED_BASE + 1):
+    """One 8^3 target sub-grid with its 26 non-leaf neighbours (24^3 frame,
+    dx = 1/24)."""
+    lv = uniform_level((3, 3, 3), ((1, 2), (1, 2), (1, 2)), seed, dx=1.0 / 24, name="single-subgrid")
+    return lv
+
+
+def config2(seed=SEED_BASE + 2):
+    """512 non-leaf target sub-grids (8x8x8 block) inside a one-slot halo
+    shell: 10^3 slots, 80^3 cells, dx = 1/80."""
+    return uniform_level((10, 10, 10), ((1, 9),) * 3, seed, name="uniform-level-512")
+
+
+def config4(seed=SEED_BASE + 4):
+    """4096 non-leaf target sub-grids (16^3 block) inside a one-slot halo
+    shell: 18^3 slots, 144^3 cells, dx = 1/144."""
+    return uniform_level((18, 18, 18), ((1, 17),) * 3, seed, name="uniform-level-4096")
+
+
+def mixed_level(shape_slots=(18, 18, 10), seed=SEED_BASE + 3, leaf_fraction=0.4, name="mixed-level"):
+    """AMR-like level with leaf and non-leaf slots.  Density
+    rho ~ (1 + r^2/0.1^2)^-1.5 about the box centre with 5% log-normal noise;
+    the 40% of target slots with the largest noisy distance from the centre
+    are leaves (monopoles at cell centres), and so is every halo slot beyond
+    the same distance; the rest is non-leaf.  Targets are the slots inside a
+    one-slot halo shell."""
+    sx, sy, sz = shape_slots
+    dx = 1.0 / (8 * max(shape_slots))
+    ext = np.array([8 * sx * dx, 8 * sy * dx, 8 * sz * dx])
+    ctr = ext / 2.0
+
+    def density(pos):
+        r2 = ((pos - ctr) ** 2).sum(axis=-1)
+        return (1.0 + r2 / 0.01) ** -1.5
+
+    centre, mass, com, M = _lattice_moments(seed, (8 * sx, 8 * sy, 8 * sz), dx, (0.0, 0.0, 0.0),
+                                            density=density, noise_sigma=0.05)
+    nsl = sx * sy * sz
+    xb = _slot_xbase(shape_slots, dx, (0.0, 0.0, 0.0))
+    slot_ctr = xb + 3.5 * dx
+    rnoise = 1.0 + 0.05 * normal(seed + 7, np.arange(nsl, dtype=np.uint64))
+    r = np.sqrt(((slot_ctr - ctr) ** 2).sum(axis=1)) * rnoise
+    interior = np.array([1 <= s // (sy * sz) < sx - 1 and 1 <= (s // sz) % sy < sy - 1 and 1 <= s % sz < sz - 1
+                         for s in range(nsl)])
+    rt = np.sort(r[interior])[::-1]
+    nleaf = int(round(leaf_fraction * rt.size))
+    kind = np.full(nsl, _abi.SLOT_NONLEAF, dtype=np.int32)
+    if nleaf > 0:
+        kind[r >= rt[nleaf - 1]] = _abi.SLOT_LEAF
+    Ms = _to_slots(M, shape_slots, 20)
+    coms = _to_slots(com, shape_slots, 3)
+    mons = _to_slots(mass[:, None], shape_slots, 1)[:, :, 0]
+    cen = _to_slots(centre, shape_slots, 3)
+    leaf = kind == _abi.SLOT_LEAF
+    Ms[leaf] = 0.0
+    coms[leaf] = cen[leaf]
+    mons[~leaf] = 0.0
+    nb = _neighbor_table(shape_slots)
+    tg = []
+    for s in range(nsl):
+        a, b, c = s // (sy * sz), (s // sz) % sy, s % sz
+        if 1 <= a < sx - 1 and 1 <= b < sy - 1 and 1 <= c < sz - 1:
+            tg.append(s)
+    tg = np.array(tg, dtype=np.int32)
+    nbk = np.where(nb >= 0, kind[np.maximum(nb, 0)], -1)
+    is_nonleaf = kind[tg] == _abi.SLOT_NONLEAF
+    has_leaf_nb = (nbk[tg] == _abi.SLOT_LEAF).any(axis=1)
+    info = {
+        "seed": seed,
+        "targets": int(tg.size),
+        "leaf": int((~is_nonleaf).sum()),
+        "nonleaf_interior": int((is_nonleaf & ~has_leaf_nb).sum()),
+        "nonleaf_boundary": int((is_nonleaf & has_leaf_nb).sum()),
+    }
+    return Level(name=name, dx=dx, shape=tuple(shape_slots), neighbors=nb, kind=kind, xbase=xb,
+                 targets=tg, M=Ms, com=coms, mon=mons, info=info)
+
+
+def config3(seed=SEED_BASE + 3):
+    """2048 target sub-grids (16x16x8) of mixed kind inside a halo shell."""
+    return mixed_level((18, 18, 10), seed=seed, name="mixed-level-2048")
+
+
+def subgrid_view(level, slot, root=False, type_=_abi.RHO):
+    """octofmm_subgrid descriptor of one slot (keeps numpy views alive in
+    the returned holder)."""
+    sg = _abi.Subgrid()
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        return a
+
+    M = arr(level.M[slot])
+    com = arr(level.com[slot])
+    mon = arr(level.mon[slot])
+    sg.M = _abi.dptr(M)
+    sg.com = _abi.dptr(com)
+    sg.mon = _abi.dptr(mon)
+    sg.dx = level.dx
+    for k in range(3):
+        sg.xbase[k] = level.xbase[slot, k]
+    sg.type = type_
+    sg.use_root_stencil = 1 if root else 0
+    for f in range(26):
+        raw = _abi.flat_to_raw(f)
+        n = int(level.neighbors[slot, raw])
+        nbd = sg.neighbors[f]
+        if n < 0 or root:
+            nbd.kind = _abi.NEIGHBOR_EMPTY
+            continue
+        nbd.count = 512
+        if level.kind[n] == _abi.SLOT_LEAF:
+            nbd.kind = _abi.NEIGHBOR_MONOPOLE
+            nbd.mon = _abi.dptr(arr(level.mon[n]))
+        else:
+            nbd.kind = _abi.NEIGHBOR_MULTIPOLE
+            nbd.M = _abi.dptr(arr(level.M[n]))
+            nbd.com = _abi.dptr(arr(level.com[n]))
+    return sg, keep
