@@ -1,0 +1,957 @@
+// octofmm.hip -- liboctofmm: gfx950 kernels and the C-ABI of include/octofmm.h.
+//
+// Data path (one level pass):
+//   1. stage_records: the caller's AoS state (M[cell][20], com[cell][3],
+//      mon[cell]) becomes one 24-double partner record per cell in HBM,
+//      ordered coarse-cell-major / child-minor inside each 8^3 slot.  The
+//      record already holds what a partner contributes under the reference's
+//      staging rules (multipole_interaction_interface.hpp:88-281): refined
+//      cells give (M, com), leaf cells give a monopole at the geometric cell
+//      centre.
+//   2. level_kernel: one 512-lane workgroup per target sub-grid.  Wave w owns
+//      a 2x2x2 block of coarse cells, lane = (coarse cell, child), so the 8
+//      lanes of one coarse cell share every partner record they load.  Each
+//      lane walks the coarse offsets of the opening-criterion stencil and the
+//      8 children of each partner coarse cell, accumulating its 20-component
+//      local expansion (and 3-component correction) in registers, and adds
+//      them to L / writes L_c once at the end.
+//   3. root: root_kernel splits the 512 x 512 all-pairs sweep of the root
+//      sub-grid over partner chunks; root_reduce sums the chunks in a fixed
+//      order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "octofmm.h"
+#include "octofmm_internal.hpp"
+
+using namespace octofmm;
+
+// ===========================================================================
+// Kernels
+// ===========================================================================
+namespace {
+
+// Workgroups are handed to the 8 XCDs round-robin; give each XCD a contiguous
+// run of items so neighbouring sub-grids share an L2.
+__device__ __forceinline__ int xcd_item(int b, int n) {
+    const int q = n >> 3, r = n & 7;
+    const int x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+__global__ __launch_bounds__(256) void stage_records(StageArgs a) {
+    // leaf-cell centres are rounded operation by operation, as the reference does
+#pragma clang fp contract(off)
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    if (gid >= a.nslots * kCells) return;
+    const int slot = gid >> 9, cell = gid & 511;
+    const int x = cell >> 6, y = (cell >> 3) & 7, z = cell & 7;
+    const int r = (((x >> 1) << 4) | ((y >> 1) << 2) | (z >> 1)) * 8 + (((x & 1) << 2) | ((y & 1) << 1) | (z & 1));
+    double* o = a.rec + ((size_t)slot * kCells + r) * kRecDoubles;
+    const int k = a.kind[slot];
+    double v[kRecDoubles];
+    if (k == OCTOFMM_SLOT_LEAF) {
+        const double m = a.mon[(size_t)slot * kCells + cell];
+        v[0] = m;
+#pragma unroll
+        for (int i = 1; i < 20; ++i) v[i] = 0.0;
+        // geometric centre in the target's frame (multipole_interaction_interface.hpp:216-235)
+        const int* fo = a.foff + 3 * slot;
+        const double* fb = a.fxbase + 3 * slot;
+        v[20] = (double)(x + fo[0]) * a.dx + fb[0] - kInx * a.dx;
+        v[21] = (double)(y + fo[1]) * a.dx + fb[1] - kInx * a.dx;
+        v[22] = (double)(z + fo[2]) * a.dx + fb[2] - kInx * a.dx;
+        v[23] = m;
+    } else if (k == OCTOFMM_SLOT_NONLEAF) {
+        const double* Mi = a.M + ((size_t)slot * kCells + cell) * 20;
+#pragma unroll
+        for (int i = 0; i < 20; ++i) v[i] = Mi[i];
+        const double* ci = a.com + ((size_t)slot * kCells + cell) * 3;
+        v[20] = ci[0];
+        v[21] = ci[1];
+        v[22] = ci[2];
+        v[23] = 0.0;
+    } else {
+#pragma unroll
+        for (int i = 0; i < kRecDoubles; ++i) v[i] = 0.0;
+    }
+    double2* o2 = reinterpret_cast<double2*>(o);
+#pragma unroll
+    for (int i = 0; i < kRecDoubles / 2; ++i) o2[i] = make_double2(v[2 * i], v[2 * i + 1]);
+}
+
+struct LaneGeom {
+    int Cx, Cy, Cz;   // coarse cell in the sub-grid (0..3)
+    int k;            // child index
+    int cell;         // x*64 + y*8 + z
+    int rec;          // record index inside the slot
+};
+
+__device__ __forceinline__ LaneGeom lane_geom() {
+    LaneGeom g;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int c = l >> 3;
+    g.k = l & 7;
+    g.Cx = ((w >> 2) & 1) * 2 + ((c >> 2) & 1);
+    g.Cy = ((w >> 1) & 1) * 2 + ((c >> 1) & 1);
+    g.Cz = (w & 1) * 2 + (c & 1);
+    const int fx = 2 * g.Cx + ((g.k >> 2) & 1), fy = 2 * g.Cy + ((g.k >> 1) & 1), fz = 2 * g.Cz + (g.k & 1);
+    g.cell = fx * 64 + fy * 8 + fz;
+    g.rec = (g.Cx * 16 + g.Cy * 4 + g.Cz) * 8 + g.k;
+    return g;
+}
+
+// Partner coarse cell C + delta -> (slot, coarse index in the slot); slot < 0
+// when the direction is empty.
+__device__ __forceinline__ int partner_base(const LaneGeom& g, int ax, int ay, int az, const int* s_nbr,
+    int& nslot, int& raw) {
+    const int px = g.Cx + ax, py = g.Cy + ay, pz = g.Cz + az;
+    const int rx = px < 0 ? 0 : (px < 4 ? 1 : 2);
+    const int ry = py < 0 ? 0 : (py < 4 ? 1 : 2);
+    const int rz = pz < 0 ? 0 : (pz < 4 ? 1 : 2);
+    raw = rx + 3 * ry + 9 * rz;
+    nslot = s_nbr[raw];
+    return ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
+}
+
+template <bool RHO>
+__device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind) {
+    const LaneGeom g = lane_geom();
+    const Tables* __restrict__ tab = a.tab;
+    const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
+    const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
+    const double mc0 = trec[0];
+    double mc[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
+    double L[20], C[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 20; ++i) L[i] = 0.0;
+
+    const int nd = tab->ndelta;
+    for (int di = 0; di < nd; ++di) {
+        const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+        unsigned fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
+        int ns, raw;
+        const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
+        if (ns < 0) fb = 0;
+        const double* pb = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
+#pragma unroll 1
+        for (int j = 0; j < 8; ++j) {
+            if (fb & (1u << j)) m2l_far<RHO>(pb + j * kRecDoubles, X0, X1, X2, mc0, mc, L, C);
+        }
+    }
+    if (flags & kItemNear) {
+        for (int di = 0; di < nd; ++di) {
+            const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+            unsigned nb = (unsigned)(tab->nearbits[di] >> (8 * g.k)) & 0xffu;
+            int ns, raw;
+            const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
+            if (ns < 0 || s_kind[raw] != OCTOFMM_SLOT_LEAF) nb = 0;
+            const double* pb = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
+#pragma unroll 1
+            for (int j = 0; j < 8; ++j) {
+                if (nb & (1u << j)) {
+                    const double* p = pb + j * kRecDoubles;
+                    m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, mc0, mc, L, C);
+                }
+            }
+        }
+    }
+    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
+#pragma unroll
+    for (int i = 0; i < 20; ++i) Lo[i] += L[i];
+    if (RHO) {
+        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
+        Co[0] = C[0];
+        Co[1] = C[1];
+        Co[2] = C[2];
+    }
+}
+
+template <bool RHO>
+__device__ void leaf_item(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind) {
+    const LaneGeom g = lane_geom();
+    const Tables* __restrict__ tab = a.tab;
+    const double* xc = a.com + ((size_t)slot * kCells + g.cell) * 3;
+    const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
+    const bool p2m = (flags & kItemP2M) != 0;
+    double L[4] = {0.0, 0.0, 0.0, 0.0}, C[3] = {0.0, 0.0, 0.0};
+    const int kx = (g.k >> 2) & 1, ky = (g.k >> 1) & 1, kz = g.k & 1;
+    const int nd = tab->ndelta;
+    for (int di = 0; di < nd; ++di) {
+        const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+        unsigned bits = (unsigned)((tab->farbits[di] | tab->nearbits[di]) >> (8 * g.k)) & 0xffu;
+        int ns, raw;
+        const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
+        const int nk = ns < 0 ? -1 : s_kind[raw];
+        if (ns < 0 || (nk != OCTOFMM_SLOT_LEAF && !p2m)) bits = 0;
+        const double* pb = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
+#pragma unroll 1
+        for (int j = 0; j < 8; ++j) {
+            if (!(bits & (1u << j))) continue;
+            const double* p = pb + j * kRecDoubles;
+            if (nk == OCTOFMM_SLOT_LEAF) {
+                // p2p (monopole_kernel_templates.hpp:16-23, kokkos_kernel.hpp:255-284)
+                const int dx = 2 * ax + ((j >> 2) & 1) - kx, dy = 2 * ay + ((j >> 1) & 1) - ky,
+                          dz = 2 * az + (j & 1) - kz;
+                const double* f = tab->four[(dx + 5) * 121 + (dy + 5) * 11 + (dz + 5)];
+                const double m = p[kRecMon] * a.inv_dx;
+                L[0] += f[0] * m;
+                L[1] += f[1] * m * a.neg_inv_dx;
+                L[2] += f[2] * m * a.neg_inv_dx;
+                L[3] += f[3] * m * a.neg_inv_dx;
+            } else {
+                p2m_pair<RHO>(p, X0, X1, X2, L, C);
+            }
+        }
+    }
+    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Lo[i] += L[i];
+    if (RHO && p2m) {
+        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
+        Co[0] = C[0];
+        Co[1] = C[1];
+        Co[2] = C[2];
+    }
+}
+
+template <bool RHO>
+__global__ __launch_bounds__(512) void level_kernel(LevelArgs a) {
+    __shared__ int s_nbr[27];
+    __shared__ int s_kind[27];
+    const int item = xcd_item(blockIdx.x, a.nitems);
+    const int2 it = a.items[item];
+    if (threadIdx.x < 27) {
+        const int n = a.nbr[(size_t)it.x * 27 + threadIdx.x];
+        s_nbr[threadIdx.x] = n;
+        s_kind[threadIdx.x] = n >= 0 ? a.kind[n] : -1;
+    }
+    __syncthreads();
+    if (it.y & kItemLeaf)
+        leaf_item<RHO>(a, it.x, it.y, s_nbr, s_kind);
+    else
+        m2l_item<RHO>(a, it.x, it.y, s_nbr, s_kind);
+}
+
+// Root sub-grid: every pair of its 512 cells except self and near offsets
+// (kokkos_kernel.hpp:324-471).  Workgroup q takes partners [q*per, (q+1)*per);
+// all lanes read the same partner, one target cell per lane.
+template <bool RHO>
+__global__ __launch_bounds__(512) void root_kernel(RootArgs a) {
+    const int t = threadIdx.x;
+    const int tx = t >> 6, ty = (t >> 3) & 7, tz = t & 7;
+    const double* trec = a.rec + (size_t)record_of_cell(tx, ty, tz) * kRecDoubles;
+    const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
+    const double mc0 = trec[0];
+    double mc[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
+    double L[20], C[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 20; ++i) L[i] = 0.0;
+    const int p0 = blockIdx.x * a.per_chunk;
+    for (int p = p0; p < p0 + a.per_chunk; ++p) {
+        const int px = p >> 6, py = (p >> 3) & 7, pz = p & 7;
+        const int dx = px - tx, dy = py - ty, dz = pz - tz;
+        const bool inbox = dx >= -5 && dx <= 5 && dy >= -5 && dy <= 5 && dz >= -5 && dz <= 5;
+        if (inbox && a.tab->root_excluded[(dx + 5) * 121 + (dy + 5) * 11 + (dz + 5)]) continue;
+        m2l_far<RHO>(a.rec + (size_t)record_of_cell(px, py, pz) * kRecDoubles, X0, X1, X2, mc0, mc, L, C);
+    }
+    double* o = a.partial + ((size_t)blockIdx.x * kCells + t) * kRecDoubles;
+#pragma unroll
+    for (int i = 0; i < 20; ++i) o[i] = L[i];
+    o[20] = C[0];
+    o[21] = C[1];
+    o[22] = C[2];
+}
+
+__global__ __launch_bounds__(512) void root_reduce(RootReduceArgs a) {
+    const int t = threadIdx.x;
+    double s[23];
+#pragma unroll
+    for (int i = 0; i < 23; ++i) s[i] = 0.0;
+    for (int q = 0; q < a.nchunks; ++q) {
+        const double* p = a.partial + ((size_t)q * kCells + t) * kRecDoubles;
+#pragma unroll
+        for (int i = 0; i < 23; ++i) s[i] += p[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 20; ++i) a.L[(size_t)t * 20 + i] += s[i];
+    if (a.rho) {
+        a.Lc[(size_t)t * 3 + 0] = s[20];
+        a.Lc[(size_t)t * 3 + 1] = s[21];
+        a.Lc[(size_t)t * 3 + 2] = s[22];
+    }
+}
+
+// Fill kernel used instead of hipMemsetAsync so the passes stay graph-safe.
+__global__ void zero_fill(double* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+}   // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+struct octofmm_context {
+    int device = 0;
+    double theta = 0.5;
+    Tables tab;
+    Tables* d_tab = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // per-sub-grid workspace: a 27-slot level (slot = raw direction index)
+    bool ws_ready = false;
+    double* h_M = nullptr;     // pinned [27][512][20]
+    double* h_com = nullptr;   // pinned [27][512][3]
+    double* h_mon = nullptr;   // pinned [27][512]
+    double* h_out = nullptr;   // pinned [512][23]
+    int32_t* h_meta = nullptr; // pinned: items[2] nbr[27*27] kind[27] foff[27*3]
+    double* h_fx = nullptr;    // pinned [27][3]
+    double* d_M = nullptr;
+    double* d_com = nullptr;
+    double* d_mon = nullptr;
+    double* d_rec = nullptr;
+    double* d_L = nullptr;     // [27][512][20]
+    double* d_Lc = nullptr;    // [27][512][3]
+    int32_t* d_meta = nullptr;
+    double* d_fx = nullptr;
+    double* d_partial = nullptr;   // root chunks
+};
+
+struct octofmm_plan {
+    octofmm_context* ctx = nullptr;
+    int32_t nslots = 0, nitems = 0, type = 0, precision = 0, flags = 0;
+    double dx = 0.0;
+    const double* d_M = nullptr;
+    const double* d_com = nullptr;
+    const double* d_mon = nullptr;
+    double* d_L = nullptr;
+    double* d_Lc = nullptr;
+    double* d_rec = nullptr;
+    int32_t* d_nbr = nullptr;
+    int32_t* d_kind = nullptr;
+    int32_t* d_foff = nullptr;
+    double* d_fxbase = nullptr;
+    int2* d_items = nullptr;
+    octofmm_plan_stats stats;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+constexpr int kRootChunks = 64;
+// layout of the per-sub-grid workspace metadata (int32 words); the int2 work
+// item comes first so it stays 8-byte aligned
+constexpr int kMetaItems = 0;
+constexpr int kMetaNbr = 2;
+constexpr int kMetaKind = kMetaNbr + 27 * 27;
+constexpr int kMetaFoff = kMetaKind + 27;
+constexpr int kMetaSize = kMetaFoff + 81;
+
+octofmm_status fail(octofmm_context* c, octofmm_status s, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return s;
+}
+
+#define HIPC(ctx, call)                                                                         \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? OCTOFMM_ERR_ALLOCATION : OCTOFMM_ERR_HIP, \
+                "%s failed: %s", #call, hipGetErrorString(e_));                                 \
+    } while (0)
+
+int raw_of_flat(int f) { return f < 13 ? f : f + 1; }
+
+octofmm_status ensure_workspace(octofmm_context* c) {
+    if (c->ws_ready) return OCTOFMM_OK;
+    HIPC(c, hipHostMalloc((void**)&c->h_M, sizeof(double) * 27 * kCells * 20, hipHostMallocDefault));
+    HIPC(c, hipHostMalloc((void**)&c->h_com, sizeof(double) * 27 * kCells * 3, hipHostMallocDefault));
+    HIPC(c, hipHostMalloc((void**)&c->h_mon, sizeof(double) * 27 * kCells, hipHostMallocDefault));
+    HIPC(c, hipHostMalloc((void**)&c->h_out, sizeof(double) * kCells * 23, hipHostMallocDefault));
+    HIPC(c, hipHostMalloc((void**)&c->h_meta, sizeof(int32_t) * kMetaSize, hipHostMallocDefault));
+    HIPC(c, hipHostMalloc((void**)&c->h_fx, sizeof(double) * 81, hipHostMallocDefault));
+    HIPC(c, hipMalloc((void**)&c->d_M, sizeof(double) * 27 * kCells * 20));
+    HIPC(c, hipMalloc((void**)&c->d_com, sizeof(double) * 27 * kCells * 3));
+    HIPC(c, hipMalloc((void**)&c->d_mon, sizeof(double) * 27 * kCells));
+    HIPC(c, hipMalloc((void**)&c->d_rec, sizeof(double) * 27 * kCells * kRecDoubles));
+    HIPC(c, hipMalloc((void**)&c->d_L, sizeof(double) * 27 * kCells * 20));
+    HIPC(c, hipMalloc((void**)&c->d_Lc, sizeof(double) * 27 * kCells * 3));
+    HIPC(c, hipMalloc((void**)&c->d_meta, sizeof(int32_t) * kMetaSize));
+    HIPC(c, hipMalloc((void**)&c->d_fx, sizeof(double) * 81));
+    HIPC(c, hipMalloc((void**)&c->d_partial, sizeof(double) * kRootChunks * kCells * kRecDoubles));
+    c->ws_ready = true;
+    return OCTOFMM_OK;
+}
+
+bool check_neighbor(const octofmm_neighbor& nb, bool want_multipole) {
+    if (nb.count < 0 || nb.count > kCells) return false;
+    if (nb.count != kCells && nb.count > 0 && !nb.cells) return false;
+    if (nb.cells)
+        for (int i = 0; i < nb.count; ++i)
+            if (nb.cells[i] < 0 || nb.cells[i] >= kCells) return false;
+    (void)want_multipole;
+    return true;
+}
+
+// Fills the 27-slot staging buffers from a sub-grid descriptor.  Partial
+// halos are expanded to full slots (zeros where no data was handed over),
+// which is what update_input does with its zero padding.
+octofmm_status fill_workspace(octofmm_context* c, const octofmm_subgrid* in, bool leaf_target,
+    bool root) {
+    int32_t* nbr = c->h_meta + kMetaNbr;  // only row 13 is used
+    int32_t* kind = c->h_meta + kMetaKind;
+    int32_t* foff = kind + 27;
+    for (int i = 0; i < 27 * 27; ++i) nbr[i] = -1;
+    for (int s = 0; s < 27; ++s) {
+        kind[s] = -1;
+        const int d[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
+        for (int k = 0; k < 3; ++k) {
+            foff[3 * s + k] = kInx + kInx * d[k];
+            c->h_fx[3 * s + k] = in->xbase[k];
+        }
+    }
+    double* M13 = c->h_M + 13 * kCells * 20;
+    double* C13 = c->h_com + 13 * kCells * 3;
+    double* m13 = c->h_mon + 13 * kCells;
+    if (leaf_target) {
+        std::memset(M13, 0, sizeof(double) * kCells * 20);
+        std::memcpy(m13, in->mon, sizeof(double) * kCells);
+        std::memcpy(C13, in->com, sizeof(double) * kCells * 3);
+        kind[13] = OCTOFMM_SLOT_LEAF;
+    } else {
+        std::memcpy(M13, in->M, sizeof(double) * kCells * 20);
+        std::memcpy(C13, in->com, sizeof(double) * kCells * 3);
+        std::memset(m13, 0, sizeof(double) * kCells);
+        kind[13] = OCTOFMM_SLOT_NONLEAF;
+    }
+    nbr[13 * 27 + 13] = 13;
+    if (root) return OCTOFMM_OK;
+    for (int f = 0; f < 26; ++f) {
+        const octofmm_neighbor& nb = in->neighbors[f];
+        const int s = raw_of_flat(f);
+        double* Ms = c->h_M + (size_t)s * kCells * 20;
+        double* Cs = c->h_com + (size_t)s * kCells * 3;
+        double* ms = c->h_mon + (size_t)s * kCells;
+        if (nb.kind == OCTOFMM_NEIGHBOR_MULTIPOLE && nb.M && nb.com) {
+            if (nb.count == kCells && !nb.cells) {
+                std::memcpy(Ms, nb.M, sizeof(double) * kCells * 20);
+                std::memcpy(Cs, nb.com, sizeof(double) * kCells * 3);
+            } else {
+                std::memset(Ms, 0, sizeof(double) * kCells * 20);
+                std::memset(Cs, 0, sizeof(double) * kCells * 3);
+                for (int i = 0; i < nb.count; ++i) {
+                    std::memcpy(Ms + 20 * nb.cells[i], nb.M + 20 * i, sizeof(double) * 20);
+                    std::memcpy(Cs + 3 * nb.cells[i], nb.com + 3 * i, sizeof(double) * 3);
+                }
+            }
+            std::memset(ms, 0, sizeof(double) * kCells);
+            kind[s] = OCTOFMM_SLOT_NONLEAF;
+            nbr[13 * 27 + s] = s;
+        } else if (nb.kind == OCTOFMM_NEIGHBOR_MONOPOLE && nb.mon) {
+            if (nb.count == kCells && !nb.cells) {
+                std::memcpy(ms, nb.mon, sizeof(double) * kCells);
+            } else {
+                std::memset(ms, 0, sizeof(double) * kCells);
+                for (int i = 0; i < nb.count; ++i) ms[nb.cells[i]] = nb.mon[i];
+            }
+            kind[s] = OCTOFMM_SLOT_LEAF;
+            nbr[13 * 27 + s] = s;
+        } else if (leaf_target && nb.kind == OCTOFMM_NEIGHBOR_MULTIPOLE) {
+            // refined neighbour without data: the reference skips its p2m
+            kind[s] = -1;
+        }
+    }
+    return OCTOFMM_OK;
+}
+
+// Uploads the staged 27-slot level and its tables.
+octofmm_status upload_workspace(octofmm_context* c, int item_flags) {
+    int32_t* meta = c->h_meta;
+    int32_t* items = meta + kMetaItems;
+    items[0] = 13;
+    items[1] = item_flags;
+    hipStream_t st = c->stream;
+    // slots actually used (13 + present neighbours) are copied whole
+    const int32_t* kind = meta + kMetaKind;
+    for (int s = 0; s < 27; ++s) {
+        if (kind[s] < 0) continue;
+        HIPC(c, hipMemcpyAsync(c->d_M + (size_t)s * kCells * 20, c->h_M + (size_t)s * kCells * 20,
+            sizeof(double) * kCells * 20, hipMemcpyHostToDevice, st));
+        HIPC(c, hipMemcpyAsync(c->d_com + (size_t)s * kCells * 3, c->h_com + (size_t)s * kCells * 3,
+            sizeof(double) * kCells * 3, hipMemcpyHostToDevice, st));
+        HIPC(c, hipMemcpyAsync(c->d_mon + (size_t)s * kCells, c->h_mon + (size_t)s * kCells,
+            sizeof(double) * kCells, hipMemcpyHostToDevice, st));
+    }
+    HIPC(c, hipMemcpyAsync(c->d_meta, meta, sizeof(int32_t) * kMetaSize, hipMemcpyHostToDevice, st));
+    HIPC(c, hipMemcpyAsync(c->d_fx, c->h_fx, sizeof(double) * 81, hipMemcpyHostToDevice, st));
+    return OCTOFMM_OK;
+}
+
+}   // namespace
+
+// Runs one sub-grid through the 27-slot workspace and leaves L (20) and
+// L_c (3) per cell in h_out.
+static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, int item_flags, bool root) {
+    octofmm_status s = upload_workspace(c, item_flags);
+    if (s != OCTOFMM_OK) return s;
+    hipStream_t st = c->stream;
+    const int rho = type == OCTOFMM_RHO;
+    StageArgs sa;
+    sa.M = c->d_M;
+    sa.com = c->d_com;
+    sa.mon = c->d_mon;
+    sa.kind = c->d_meta + kMetaKind;
+    sa.fxbase = c->d_fx;
+    sa.foff = c->d_meta + kMetaFoff;
+    sa.rec = c->d_rec;
+    sa.dx = dx;
+    sa.nslots = 27;
+    hipLaunchKernelGGL(stage_records, dim3(27 * kCells / 256), dim3(256), 0, st, sa);
+    HIPC(c, hipGetLastError());
+    const size_t nL = (size_t)kCells * 20, nC = (size_t)kCells * 3;
+    double* L13 = c->d_L + 13 * nL;
+    double* C13 = c->d_Lc + 13 * nC;
+    hipLaunchKernelGGL(zero_fill, dim3((nL + 255) / 256), dim3(256), 0, st, L13, nL);
+    hipLaunchKernelGGL(zero_fill, dim3((nC + 255) / 256), dim3(256), 0, st, C13, nC);
+    if (root) {
+        RootArgs ra;
+        ra.rec = c->d_rec + (size_t)13 * kCells * kRecDoubles;
+        ra.tab = c->d_tab;
+        ra.partial = c->d_partial;
+        ra.per_chunk = kCells / kRootChunks;
+        ra.nchunks = kRootChunks;
+        if (rho)
+            hipLaunchKernelGGL(root_kernel<true>, dim3(kRootChunks), dim3(512), 0, st, ra);
+        else
+            hipLaunchKernelGGL(root_kernel<false>, dim3(kRootChunks), dim3(512), 0, st, ra);
+        HIPC(c, hipGetLastError());
+        RootReduceArgs rr;
+        rr.partial = c->d_partial;
+        rr.L = L13;
+        rr.Lc = C13;
+        rr.nchunks = kRootChunks;
+        rr.rho = rho;
+        hipLaunchKernelGGL(root_reduce, dim3(1), dim3(512), 0, st, rr);
+        HIPC(c, hipGetLastError());
+    } else {
+        LevelArgs la;
+        la.rec = c->d_rec;
+        la.com = c->d_com;
+        la.L = c->d_L;
+        la.Lc = c->d_Lc;
+        la.nbr = c->d_meta + kMetaNbr;
+        la.kind = c->d_meta + kMetaKind;
+        la.items = reinterpret_cast<const int2*>(c->d_meta + kMetaItems);
+        la.tab = c->d_tab;
+        la.nitems = 1;
+        la.inv_dx = 1.0 / dx;
+        la.neg_inv_dx = -1.0 / dx;
+        if (rho)
+            hipLaunchKernelGGL(level_kernel<true>, dim3(1), dim3(512), 0, st, la);
+        else
+            hipLaunchKernelGGL(level_kernel<false>, dim3(1), dim3(512), 0, st, la);
+        HIPC(c, hipGetLastError());
+    }
+    // results: L (20) then L_c (3) per cell into h_out
+    HIPC(c, hipMemcpyAsync(c->h_out, L13, sizeof(double) * nL, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipMemcpyAsync(c->h_out + nL, C13, sizeof(double) * nC, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipStreamSynchronize(st));
+    return OCTOFMM_OK;
+}
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int32_t octofmm_abi_version(void) { return OCTOFMM_ABI_VERSION; }
+
+const char* octofmm_status_string(octofmm_status s) {
+    switch (s) {
+    case OCTOFMM_OK: return "ok";
+    case OCTOFMM_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case OCTOFMM_ERR_THETA: return "invalid theta / stencil";
+    case OCTOFMM_ERR_NO_DEVICE: return "no HIP device";
+    case OCTOFMM_ERR_HIP: return "HIP runtime error";
+    case OCTOFMM_ERR_ALLOCATION: return "allocation failed";
+    case OCTOFMM_ERR_SIZE: return "size not supported";
+    case OCTOFMM_ERR_NONFINITE: return "non-finite input";
+    }
+    return "unknown status";
+}
+
+const char* octofmm_last_error(const octofmm_context* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+octofmm_status octofmm_create(const octofmm_config* cfg, octofmm_context** out) {
+    if (!cfg || !out) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return OCTOFMM_ERR_NO_DEVICE;
+    if (cfg->device < 0 || cfg->device >= n) return OCTOFMM_ERR_NO_DEVICE;
+    octofmm_context* c = new octofmm_context();
+    c->device = cfg->device;
+    c->theta = cfg->theta;
+    char err[256];
+    if (!build_tables(cfg->theta, c->tab, err, sizeof(err))) {
+        delete c;
+        return OCTOFMM_ERR_THETA;
+    }
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&c->d_tab, sizeof(Tables)) != hipSuccess ||
+        hipMemcpy(c->d_tab, &c->tab, sizeof(Tables), hipMemcpyHostToDevice) != hipSuccess) {
+        octofmm_destroy(c);
+        return OCTOFMM_ERR_HIP;
+    }
+    *out = c;
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_destroy(octofmm_context* c) {
+    if (!c) return OCTOFMM_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipHostFree(c->h_M);
+    (void)hipHostFree(c->h_com);
+    (void)hipHostFree(c->h_mon);
+    (void)hipHostFree(c->h_out);
+    (void)hipHostFree(c->h_meta);
+    (void)hipHostFree(c->h_fx);
+    (void)hipFree(c->d_M);
+    (void)hipFree(c->d_com);
+    (void)hipFree(c->d_mon);
+    (void)hipFree(c->d_rec);
+    (void)hipFree(c->d_L);
+    (void)hipFree(c->d_Lc);
+    (void)hipFree(c->d_meta);
+    (void)hipFree(c->d_fx);
+    (void)hipFree(c->d_partial);
+    (void)hipFree(c->d_tab);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_compute_multipole_interactions(octofmm_context* c, const octofmm_subgrid* in,
+    double* L, double* L_c) {
+    if (!c) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    if (!in || !in->M || !in->com || !L) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "null input or output");
+    if (in->type != OCTOFMM_RHO && in->type != OCTOFMM_DRHODT) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad type");
+    if (in->type == OCTOFMM_RHO && !L_c) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "RHO needs L_c");
+    const bool root = in->use_root_stencil != 0;
+    int flags = 0;
+    if (!root)
+        for (int f = 0; f < 26; ++f) {
+            const octofmm_neighbor& nb = in->neighbors[f];
+            if (nb.kind < 0 || nb.kind > 2) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad neighbour kind %d", f);
+            if (nb.kind != OCTOFMM_NEIGHBOR_EMPTY && !check_neighbor(nb, nb.kind == OCTOFMM_NEIGHBOR_MULTIPOLE))
+                return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad neighbour %d", f);
+            if (nb.kind == OCTOFMM_NEIGHBOR_MONOPOLE && nb.mon) flags |= kItemNear;
+        }
+    HIPC(c, hipSetDevice(c->device));
+    octofmm_status s = ensure_workspace(c);
+    if (s != OCTOFMM_OK) return s;
+    s = fill_workspace(c, in, false, root);
+    if (s != OCTOFMM_OK) return s;
+    s = launch_subgrid(c, in->dx, in->type, flags, root);
+    if (s != OCTOFMM_OK) return s;
+    const double* r = c->h_out;
+    for (int i = 0; i < kCells * 20; ++i) L[i] += r[i];
+    if (in->type == OCTOFMM_RHO)
+        for (int i = 0; i < kCells * 3; ++i) L_c[i] = r[kCells * 20 + i];
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_compute_monopole_interactions(octofmm_context* c, const octofmm_subgrid* in,
+    int32_t contains_multipole_neighbor, double* L, double* L_c) {
+    if (!c) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    if (!in || !in->mon || !in->com || !L) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "null input or output");
+    if (in->type != OCTOFMM_RHO && in->type != OCTOFMM_DRHODT) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad type");
+    const bool p2m = contains_multipole_neighbor != 0;
+    if (in->type == OCTOFMM_RHO && p2m && !L_c) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "RHO with p2m needs L_c");
+    for (int f = 0; f < 26; ++f) {
+        const octofmm_neighbor& nb = in->neighbors[f];
+        if (nb.kind < 0 || nb.kind > 2) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad neighbour kind %d", f);
+        if (nb.kind != OCTOFMM_NEIGHBOR_EMPTY && !check_neighbor(nb, nb.kind == OCTOFMM_NEIGHBOR_MULTIPOLE))
+            return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad neighbour %d", f);
+    }
+    HIPC(c, hipSetDevice(c->device));
+    octofmm_status s = ensure_workspace(c);
+    if (s != OCTOFMM_OK) return s;
+    s = fill_workspace(c, in, true, false);
+    if (s != OCTOFMM_OK) return s;
+    s = launch_subgrid(c, in->dx, in->type, kItemLeaf | (p2m ? kItemP2M : 0), false);
+    if (s != OCTOFMM_OK) return s;
+    const double* r = c->h_out;
+    for (int u = 0; u < kCells; ++u)
+        for (int i = 0; i < 4; ++i) L[20 * u + i] += r[20 * u + i];
+    if (in->type == OCTOFMM_RHO && p2m)
+        for (int i = 0; i < kCells * 3; ++i) L_c[i] = r[kCells * 20 + i];
+    return OCTOFMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Level plans
+// ---------------------------------------------------------------------------
+static void plan_free(octofmm_plan* p) {
+    if (!p) return;
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->d_rec);
+    (void)hipFree(p->d_nbr);
+    (void)hipFree(p->d_kind);
+    (void)hipFree(p->d_foff);
+    (void)hipFree(p->d_fxbase);
+    (void)hipFree(p->d_items);
+    delete p;
+}
+
+static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mask) {
+    octofmm_context* c = p->ctx;
+    if (stage_mask & 1) {
+        StageArgs sa;
+        sa.M = p->d_M;
+        sa.com = p->d_com;
+        sa.mon = p->d_mon;
+        sa.kind = p->d_kind;
+        sa.fxbase = p->d_fxbase;
+        sa.foff = p->d_foff;
+        sa.rec = p->d_rec;
+        sa.dx = p->dx;
+        sa.nslots = p->nslots;
+        const int nthreads = p->nslots * kCells;
+        hipLaunchKernelGGL(stage_records, dim3((nthreads + 255) / 256), dim3(256), 0, st, sa);
+        HIPC(c, hipGetLastError());
+    }
+    if ((stage_mask & 2) && p->nitems > 0) {
+        LevelArgs la;
+        la.rec = p->d_rec;
+        la.com = p->d_com;
+        la.L = p->d_L;
+        la.Lc = p->d_Lc;
+        la.nbr = p->d_nbr;
+        la.kind = p->d_kind;
+        la.items = p->d_items;
+        la.tab = c->d_tab;
+        la.nitems = p->nitems;
+        la.inv_dx = 1.0 / p->dx;
+        la.neg_inv_dx = -1.0 / p->dx;
+        if (p->type == OCTOFMM_RHO)
+            hipLaunchKernelGGL(level_kernel<true>, dim3(p->nitems), dim3(512), 0, st, la);
+        else
+            hipLaunchKernelGGL(level_kernel<false>, dim3(p->nitems), dim3(512), 0, st, la);
+        HIPC(c, hipGetLastError());
+    }
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level* lv, octofmm_plan** out) {
+    if (!c) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    if (!lv || !out) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "null level or plan");
+    *out = nullptr;
+    if (lv->nslots <= 0 || !lv->neighbors || !lv->kind || !lv->xbase || lv->ntargets < 0 ||
+        (lv->ntargets > 0 && !lv->targets))
+        return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad level description");
+    if (!lv->d_M || !lv->d_com || !lv->d_mon || !lv->d_L || (lv->type == OCTOFMM_RHO && !lv->d_L_c))
+        return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "null device array");
+    if (lv->type != OCTOFMM_RHO && lv->type != OCTOFMM_DRHODT) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad type");
+    if (lv->precision != OCTOFMM_PRECISION_FP64 && lv->precision != OCTOFMM_PRECISION_MIXED)
+        return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad precision");
+    if (!(lv->dx > 0.0)) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "dx must be positive");
+    if ((size_t)lv->nslots * kCells * kRecDoubles * sizeof(double) >= ((size_t)1 << 40))
+        return fail(c, OCTOFMM_ERR_SIZE, "level too large");
+    const int ns = lv->nslots;
+    for (int s = 0; s < ns; ++s) {
+        if (lv->kind[s] != OCTOFMM_SLOT_NONLEAF && lv->kind[s] != OCTOFMM_SLOT_LEAF)
+            return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad kind of slot %d", s);
+        for (int r = 0; r < 27; ++r) {
+            const int n = lv->neighbors[(size_t)s * 27 + r];
+            if (n < -1 || n >= ns) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad neighbour of slot %d", s);
+        }
+        if (lv->neighbors[(size_t)s * 27 + 13] != s)
+            return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "neighbour entry 13 of slot %d must be the slot", s);
+    }
+    // work items, heaviest kinds first: non-leaf with leaf neighbours, non-leaf,
+    // leaf with refined neighbours, pure p2p leaves
+    std::vector<int2> buckets[4];
+    octofmm_plan_stats st;
+    std::memset(&st, 0, sizeof(st));
+    const Tables& T = c->tab;
+    std::vector<char> seen(ns, 0);
+    for (int t = 0; t < lv->ntargets; ++t) {
+        const int s = lv->targets[t];
+        if (s < 0 || s >= ns) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad target %d", t);
+        if (seen[s]) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "duplicate target slot %d", s);
+        seen[s] = 1;
+        bool has_leaf = false, has_nonleaf = false;
+        for (int r = 0; r < 27; ++r) {
+            if (r == 13) continue;
+            const int n = lv->neighbors[(size_t)s * 27 + r];
+            if (n < 0) continue;
+            (lv->kind[n] == OCTOFMM_SLOT_LEAF ? has_leaf : has_nonleaf) = true;
+        }
+        if (lv->kind[s] == OCTOFMM_SLOT_NONLEAF) {
+            buckets[has_leaf ? 0 : 1].push_back(make_int2(s, has_leaf ? kItemNear : 0));
+            st.n_m2l_items++;
+        } else {
+            buckets[has_nonleaf ? 2 : 3].push_back(make_int2(s, kItemLeaf | (has_nonleaf ? kItemP2M : 0)));
+            st.n_leaf_items++;
+        }
+    }
+    // pair counts: per cell per coarse offset
+    for (int b = 0; b < 4; ++b)
+        for (const int2& it : buckets[b]) {
+            const int s = it.x;
+            for (int cx = 0; cx < 4; ++cx)
+                for (int cy = 0; cy < 4; ++cy)
+                    for (int cz = 0; cz < 4; ++cz)
+                        for (int di = 0; di < T.ndelta; ++di) {
+                            const int px = cx + T.delta[di][0], py = cy + T.delta[di][1], pz = cz + T.delta[di][2];
+                            const int raw = (px < 0 ? 0 : px < 4 ? 1 : 2) + 3 * (py < 0 ? 0 : py < 4 ? 1 : 2) +
+                                9 * (pz < 0 ? 0 : pz < 4 ? 1 : 2);
+                            const int n = lv->neighbors[(size_t)s * 27 + raw];
+                            if (n < 0) continue;
+                            const int64_t nf = __builtin_popcountll(T.farbits[di]);
+                            const int64_t nn = __builtin_popcountll(T.nearbits[di]);
+                            if (!(it.y & kItemLeaf)) {
+                                st.far_pairs += nf;
+                                if (lv->kind[n] == OCTOFMM_SLOT_LEAF) st.near_pairs += nn;
+                            } else if (lv->kind[n] == OCTOFMM_SLOT_LEAF) {
+                                st.p2p_pairs += nf + nn;
+                            } else if (it.y & kItemP2M) {
+                                st.p2m_pairs += nf + nn;
+                            }
+                        }
+        }
+    const bool rho = lv->type == OCTOFMM_RHO;
+    st.flops = st.far_pairs * (rho ? 407 : 273) + st.near_pairs * (rho ? 407 : 273) + st.p2p_pairs * 12 +
+        st.p2m_pairs * (rho ? 318 : 205);
+    std::vector<int2> items;
+    for (int b = 0; b < 4; ++b) items.insert(items.end(), buckets[b].begin(), buckets[b].end());
+
+    HIPC(c, hipSetDevice(c->device));
+    octofmm_plan* p = new octofmm_plan();
+    p->ctx = c;
+    p->nslots = ns;
+    p->nitems = (int32_t)items.size();
+    p->type = lv->type;
+    p->precision = lv->precision;
+    p->flags = lv->flags;
+    p->dx = lv->dx;
+    p->d_M = lv->d_M;
+    p->d_com = lv->d_com;
+    p->d_mon = lv->d_mon;
+    p->d_L = lv->d_L;
+    p->d_Lc = lv->d_L_c;
+    p->stats = st;
+    std::vector<int32_t> foff(3 * (size_t)ns, kInx);
+    auto cleanup = [&](octofmm_status s, const char* what) {
+        plan_free(p);
+        return fail(c, s, "%s", what);
+    };
+    if (hipMalloc((void**)&p->d_rec, (size_t)ns * kCells * kRecDoubles * sizeof(double)) != hipSuccess)
+        return cleanup(OCTOFMM_ERR_ALLOCATION, "record buffer allocation failed");
+    if (hipMalloc((void**)&p->d_nbr, sizeof(int32_t) * 27 * ns) != hipSuccess ||
+        hipMalloc((void**)&p->d_kind, sizeof(int32_t) * ns) != hipSuccess ||
+        hipMalloc((void**)&p->d_foff, sizeof(int32_t) * 3 * ns) != hipSuccess ||
+        hipMalloc((void**)&p->d_fxbase, sizeof(double) * 3 * ns) != hipSuccess ||
+        hipMalloc((void**)&p->d_items, sizeof(int2) * std::max<size_t>(1, items.size())) != hipSuccess)
+        return cleanup(OCTOFMM_ERR_ALLOCATION, "plan table allocation failed");
+    if (hipMemcpy(p->d_nbr, lv->neighbors, sizeof(int32_t) * 27 * ns, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_kind, lv->kind, sizeof(int32_t) * ns, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_foff, foff.data(), sizeof(int32_t) * 3 * ns, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_fxbase, lv->xbase, sizeof(double) * 3 * ns, hipMemcpyHostToDevice) != hipSuccess ||
+        (!items.empty() &&
+            hipMemcpy(p->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess))
+        return cleanup(OCTOFMM_ERR_HIP, "plan table upload failed");
+    if (lv->flags & OCTOFMM_PLAN_USE_GRAPH) {
+        hipStream_t cs = c->stream;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return cleanup(OCTOFMM_ERR_HIP, "graph capture failed to start");
+        octofmm_status ls = plan_launch(p, cs, 3);
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(cs, &g);
+        if (ls != OCTOFMM_OK || ee != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return cleanup(OCTOFMM_ERR_HIP, "graph capture failed");
+        }
+        p->graph = g;
+        if (hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0) != hipSuccess)
+            return cleanup(OCTOFMM_ERR_HIP, "graph instantiation failed");
+    }
+    *out = p;
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_level_plan_execute(octofmm_plan* p, void* stream) {
+    if (!p) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    octofmm_context* c = p->ctx;
+    HIPC(c, hipSetDevice(c->device));
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (p->gexec) {
+        HIPC(c, hipGraphLaunch(p->gexec, st));
+        return OCTOFMM_OK;
+    }
+    return plan_launch(p, st, 3);
+}
+
+octofmm_status octofmm_level_plan_stats(const octofmm_plan* p, octofmm_plan_stats* s) {
+    if (!p || !s) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    *s = p->stats;
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_level_plan_time_stages(octofmm_plan* p, void* stream, int32_t iters, double* ms) {
+    if (!p || !ms || iters <= 0) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    octofmm_context* c = p->ctx;
+    HIPC(c, hipSetDevice(c->device));
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    HIPC(c, hipEventCreate(&e0));
+    HIPC(c, hipEventCreate(&e1));
+    for (int stage = 0; stage < 2; ++stage) {
+        HIPC(c, hipEventRecord(e0, st));
+        for (int i = 0; i < iters; ++i) {
+            octofmm_status s = plan_launch(p, st, 1 << stage);
+            if (s != OCTOFMM_OK) return s;
+        }
+        HIPC(c, hipEventRecord(e1, st));
+        HIPC(c, hipEventSynchronize(e1));
+        float f = 0.f;
+        HIPC(c, hipEventElapsedTime(&f, e0, e1));
+        ms[stage] = (double)f / iters;
+    }
+    ms[2] = 0.0;   // write-back is fused into the interaction kernel
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_level_plan_destroy(octofmm_plan* p) {
+    if (!p) return OCTOFMM_OK;
+    (void)hipSetDevice(p->ctx->device);
+    plan_free(p);
+    return OCTOFMM_OK;
+}
+
+}   // extern "C"

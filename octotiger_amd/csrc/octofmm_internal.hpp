@@ -1,0 +1,56 @@
+// octofmm_internal.hpp -- definitions shared by the host-side plan builder and
+// the gfx950 kernels of liboctofmm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace octofmm {
+
+constexpr int kInx = 8;
+constexpr int kCells = 512;
+constexpr int kRecDoubles = 24;       // partner record: m0, M1..M19, Y0..Y2, mon
+constexpr int kRecY = 20;
+constexpr int kRecMon = 23;
+constexpr int kMaxDelta = 125;        // coarse offsets in [-2, 2]^3 (theta >= 0.34)
+constexpr int kStencilInx = 11;       // stencil box edge (offsets -5..5)
+constexpr int kStencilBox = kStencilInx * kStencilInx * kStencilInx;
+constexpr double kThetaFloor = 0.34;  // OCTOTIGER_THETA_MINIMUM default (CMakeLists.txt:66)
+
+// Item flags of a level plan
+constexpr int kItemLeaf = 1;          // leaf target: p2p (+ p2m)
+constexpr int kItemNear = 2;          // non-leaf target with a leaf neighbour: near pass
+constexpr int kItemP2M = 4;           // leaf target with a refined neighbour: p2m + L_c
+
+// Interaction tables derived from theta by the reference's stencil rule.
+// A target cell t = 2C + k (C its coarse cell, k its child index
+// (x&1)*4 + (y&1)*2 + (z&1)) interacts with partner p = 2(C + delta) + j for
+// every coarse offset delta that passes the opening criterion; the pair is
+// "far" (multipole partner) or "near" (monopole-only partner) by the fine
+// offset d = 2 delta + j - k.
+struct Tables {
+    int32_t ndelta;
+    int32_t max_delta;                // largest |delta| component
+    int32_t nfar_per_cell;            // far partners of one cell
+    int32_t nnear_per_cell;           // near partners of one cell
+    int8_t delta[kMaxDelta][4];       // coarse offsets (x, y, z, 0)
+    uint64_t farbits[kMaxDelta];      // byte k: bit j set if (delta, j, k) is far
+    uint64_t nearbits[kMaxDelta];     // byte k: bit j set if (delta, j, k) is near (self excluded)
+    double four[kStencilBox][4];      // p2p constants (-1/r, x/r^3, y/r^3, z/r^3) by fine offset
+    uint8_t root_excluded[kStencilBox];   // root kernel: near offsets and self
+    uint8_t phase[kStencilBox];       // 1 far, 0 near (inside the stencil), 1 elsewhere
+    uint8_t mask[kStencilBox];        // union stencil mask
+};
+
+// Builds the tables; returns false when theta is below the floor or the
+// stencil fails the reference's symmetry / width checks.
+bool build_tables(double theta, Tables& t, char* err, size_t errlen);
+
+__host__ __device__ inline int child_of(int x, int y, int z) { return ((x & 1) << 2) | ((y & 1) << 1) | (z & 1); }
+__host__ __device__ inline int coarse_of(int x, int y, int z) { return ((x >> 1) << 4) | ((y >> 1) << 2) | (z >> 1); }
+// record index of cell (x,y,z) inside its slot: coarse-major, child-minor
+__host__ __device__ inline int record_of_cell(int x, int y, int z) { return coarse_of(x, y, z) * 8 + child_of(x, y, z); }
+
+}   // namespace octofmm

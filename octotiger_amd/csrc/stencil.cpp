@@ -1,0 +1,144 @@
+// stencil.cpp -- interaction tables from the opening angle theta.
+//
+// Follows the reference's stencil rule: for each of the 8 parity positions of
+// a cell inside its parent, every offset whose coarse (parent-level) distance
+// passes the opening criterion 1/|dc| > theta enters the stencil; the offset is
+// a multipole ("far") partner when its fine distance satisfies 1/|d| <= theta
+// and a monopole-only ("near") partner otherwise; the union over the 8 parity
+// positions is the stencil mask, and it must be symmetric and no wider than the
+// stencil box compiled for theta >= 0.34
+// (src/multipole_interactions/util/calculate_stencil.cpp:23-161,
+//  src/monopole_interactions/util/calculate_stencil.cpp:19-177,
+//  octotiger/common_kernel/helper.hpp:20-31).  The kernels then apply the
+// per-pair coarse test dc^2 < (1/theta)^2 in double precision
+// (octotiger/multipole_interactions/kernel/kokkos_kernel.hpp:202-256).
+#include "octofmm_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace octofmm {
+namespace {
+
+double recip_dist(long long a0, long long a1, long long a2, long long b0, long long b1, long long b2) {
+    const double t = static_cast<double>((a0 - b0) * (a0 - b0) + (a1 - b1) * (a1 - b1) + (a2 - b2) * (a2 - b2));
+    return t > 0.0 ? 1.0 / std::sqrt(t) : 1.0e10;
+}
+
+inline int box_index(int x, int y, int z) {
+    return (x + 5) * kStencilInx * kStencilInx + (y + 5) * kStencilInx + (z + 5);
+}
+
+}   // namespace
+
+bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
+    std::memset(&t, 0, sizeof(t));
+    if (!(theta >= kThetaFloor) || !(theta < 1.0)) {
+        std::snprintf(err, errlen, "theta %g outside [%g, 1)", theta, kThetaFloor);
+        return false;
+    }
+    // stencil mask and phase over the 11^3 box; first parity position that
+    // contributes an offset decides its phase (it is a function of the offset)
+    std::vector<int> seen(kStencilBox, 0);
+    for (int i = 0; i < kStencilBox; ++i) t.phase[i] = 1;
+    int mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    for (int p = 0; p < 8; ++p) {
+        const int i0 = (p >> 2) & 1, i1 = (p >> 1) & 1, i2 = p & 1;
+        for (int j0 = i0 - kInx; j0 < i0 + kInx; ++j0)
+            for (int j1 = i1 - kInx; j1 < i1 + kInx; ++j1)
+                for (int j2 = i2 - kInx; j2 < i2 + kInx; ++j2) {
+                    if (i0 == j0 && i1 == j1 && i2 == j2) continue;
+                    const int c0 = (i0 + kInx) / 2 - kInx / 2, c1 = (i1 + kInx) / 2 - kInx / 2,
+                              c2 = (i2 + kInx) / 2 - kInx / 2;
+                    const int e0 = (j0 + kInx) / 2 - kInx / 2, e1 = (j1 + kInx) / 2 - kInx / 2,
+                              e2 = (j2 + kInx) / 2 - kInx / 2;
+                    const double tf = recip_dist(i0, i1, i2, j0, j1, j2);
+                    const double tc = recip_dist(c0, c1, c2, e0, e1, e2);
+                    if (!(tc > theta)) continue;
+                    const int d[3] = {j0 - i0, j1 - i1, j2 - i2};
+                    for (int k = 0; k < 3; ++k) {
+                        mn[k] = d[k] < mn[k] ? d[k] : mn[k];
+                        mx[k] = d[k] > mx[k] ? d[k] : mx[k];
+                    }
+                    if (d[0] < -5 || d[0] > 5 || d[1] < -5 || d[1] > 5 || d[2] < -5 || d[2] > 5) continue;
+                    const int bi = box_index(d[0], d[1], d[2]);
+                    if (!seen[bi]) {
+                        seen[bi] = 1;
+                        t.mask[bi] = 1;
+                        t.phase[bi] = (tf <= theta) ? 1 : 0;
+                    }
+                }
+    }
+    if (mx[2] != mx[1] || mx[1] != mx[0] || mn[2] != mn[1] || mn[1] != mn[0] || mn[2] != -mx[2]) {
+        std::snprintf(err, errlen, "stencil for theta %g is not symmetric", theta);
+        return false;
+    }
+    if (mx[0] > 5) {
+        std::snprintf(err, errlen, "stencil for theta %g wider than the compiled maximum 5", theta);
+        return false;
+    }
+    // p2p constants and root exclusions over the box
+    for (int x = -5; x <= 5; ++x)
+        for (int y = -5; y <= 5; ++y)
+            for (int z = -5; z <= 5; ++z) {
+                const int bi = box_index(x, y, z);
+                if (t.mask[bi]) {
+                    const double fx = x, fy = y, fz = z;
+                    const double r = std::sqrt(fx * fx + fy * fy + fz * fz);
+                    const double r3 = r * r * r;
+                    t.four[bi][0] = -1.0 / r;
+                    t.four[bi][1] = fx / r3;
+                    t.four[bi][2] = fy / r3;
+                    t.four[bi][3] = fz / r3;
+                }
+                t.root_excluded[bi] = (!t.phase[bi] || (x == 0 && y == 0 && z == 0)) ? 1 : 0;
+            }
+    // coarse offsets passing the kernels' double-precision coarse test
+    const double th2 = (1.0 / theta) * (1.0 / theta);
+    int n = 0, maxd = 0;
+    for (int ax = -2; ax <= 2; ++ax)
+        for (int ay = -2; ay <= 2; ++ay)
+            for (int az = -2; az <= 2; ++az) {
+                if (!(static_cast<double>(ax * ax + ay * ay + az * az) < th2)) continue;
+                uint64_t fb = 0, nb = 0;
+                for (int k = 0; k < 8; ++k)
+                    for (int j = 0; j < 8; ++j) {
+                        const int dx = 2 * ax + ((j >> 2) & 1) - ((k >> 2) & 1);
+                        const int dy = 2 * ay + ((j >> 1) & 1) - ((k >> 1) & 1);
+                        const int dz = 2 * az + (j & 1) - (k & 1);
+                        if (dx == 0 && dy == 0 && dz == 0) continue;
+                        if (dx < -5 || dx > 5 || dy < -5 || dy > 5 || dz < -5 || dz > 5) continue;
+                        const int bi = box_index(dx, dy, dz);
+                        if (!t.mask[bi]) continue;
+                        if (t.phase[bi])
+                            fb |= uint64_t(1) << (8 * k + j);
+                        else
+                            nb |= uint64_t(1) << (8 * k + j);
+                    }
+                if (!fb && !nb) continue;
+                t.delta[n][0] = (int8_t)ax;
+                t.delta[n][1] = (int8_t)ay;
+                t.delta[n][2] = (int8_t)az;
+                t.farbits[n] = fb;
+                t.nearbits[n] = nb;
+                const int m = std::max(std::abs(ax), std::max(std::abs(ay), std::abs(az)));
+                maxd = m > maxd ? m : maxd;
+                ++n;
+            }
+    t.ndelta = n;
+    t.max_delta = maxd;
+    int nf = 0, nn = 0;
+    for (int i = 0; i < n; ++i) {
+        nf += __builtin_popcountll(t.farbits[i] & 0xffull);
+        nn += __builtin_popcountll(t.nearbits[i] & 0xffull);
+    }
+    t.nfar_per_cell = nf;
+    t.nnear_per_cell = nn;
+    return true;
+}
+
+}   // namespace octofmm

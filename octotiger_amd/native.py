@@ -1,0 +1,159 @@
+"""ctypes binding of liboctofmm.so, the gfx950 C-ABI library (include/octofmm.h).
+
+The library is the only compute path: when it is missing, or no HIP device is
+present, every call raises -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboctofmm.so")
+
+EXPORTED = [
+    "octofmm_abi_version", "octofmm_status_string", "octofmm_last_error", "octofmm_create",
+    "octofmm_destroy", "octofmm_compute_multipole_interactions",
+    "octofmm_compute_monopole_interactions", "octofmm_level_plan_create",
+    "octofmm_level_plan_execute", "octofmm_level_plan_stats", "octofmm_level_plan_time_stages",
+    "octofmm_level_plan_destroy",
+]
+
+_lib = None
+
+
+class OctofmmError(RuntimeError):
+    def __init__(self, status, msg=""):
+        self.status = status
+        super().__init__(f"octofmm status {status}: {msg}")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (python -m octotiger_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.octofmm_abi_version.restype = ctypes.c_int32
+        L.octofmm_status_string.restype = ctypes.c_char_p
+        L.octofmm_status_string.argtypes = [ctypes.c_int]
+        L.octofmm_last_error.restype = ctypes.c_char_p
+        L.octofmm_last_error.argtypes = [vp]
+        L.octofmm_create.argtypes = [ctypes.POINTER(_abi.Config), ctypes.POINTER(vp)]
+        L.octofmm_destroy.argtypes = [vp]
+        L.octofmm_compute_multipole_interactions.argtypes = [
+            vp, ctypes.POINTER(_abi.Subgrid), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.octofmm_compute_monopole_interactions.argtypes = [
+            vp, ctypes.POINTER(_abi.Subgrid), ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+            ctypes.POINTER(ctypes.c_double)]
+        L.octofmm_level_plan_create.argtypes = [vp, ctypes.POINTER(_abi.Level), ctypes.POINTER(vp)]
+        L.octofmm_level_plan_execute.argtypes = [vp, vp]
+        L.octofmm_level_plan_stats.argtypes = [vp, ctypes.POINTER(_abi.PlanStats)]
+        L.octofmm_level_plan_time_stages.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        L.octofmm_level_plan_destroy.argtypes = [vp]
+        for fn in EXPORTED:
+            getattr(L, fn).restype = getattr(L, fn).restype or ctypes.c_int
+        L.octofmm_status_string.restype = ctypes.c_char_p
+        L.octofmm_last_error.restype = ctypes.c_char_p
+        L.octofmm_abi_version.restype = ctypes.c_int32
+        _lib = L
+    return _lib
+
+
+class Context:
+    """One GPU, one opening angle; stencils are built at creation."""
+
+    def __init__(self, theta=0.5, device=0):
+        self._h = ctypes.c_void_p()
+        cfg = _abi.Config(theta=theta, device=device)
+        st = lib().octofmm_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if st != _abi.OK:
+            raise OctofmmError(st, lib().octofmm_status_string(st).decode())
+        self.theta = theta
+        self.device = device
+
+    def _check(self, st):
+        if st != _abi.OK:
+            raise OctofmmError(st, lib().octofmm_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().octofmm_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def multipole_interactions(self, subgrid, L, L_c):
+        """L [512,20] accumulated, L_c [512,3] overwritten (RHO)."""
+        self._check(lib().octofmm_compute_multipole_interactions(
+            self._h, ctypes.byref(subgrid), _abi.dptr(L), _abi.dptr(L_c)))
+
+    def monopole_interactions(self, subgrid, contains_multipole_neighbor, L, L_c):
+        self._check(lib().octofmm_compute_monopole_interactions(
+            self._h, ctypes.byref(subgrid), 1 if contains_multipole_neighbor else 0, _abi.dptr(L),
+            _abi.dptr(L_c) if L_c is not None else None))
+
+    def level_plan(self, level, d_M, d_com, d_mon, d_L, d_L_c, type_=_abi.RHO,
+                   precision=_abi.PRECISION_FP64, mixed_far_d2=4, flags=_abi.PLAN_USE_GRAPH,
+                   targets=None):
+        """Plan over a workloads.Level whose state lives in device buffers
+        (addresses as ints, e.g. torch tensor data_ptr())."""
+        return LevelPlan(self, level, d_M, d_com, d_mon, d_L, d_L_c, type_, precision, mixed_far_d2,
+                         flags, targets)
+
+
+class LevelPlan:
+    def __init__(self, ctx, level, d_M, d_com, d_mon, d_L, d_L_c, type_, precision, mixed_far_d2, flags,
+                 targets):
+        self.ctx = ctx
+        self._keep = [np.ascontiguousarray(level.neighbors, dtype=np.int32),
+                      np.ascontiguousarray(level.kind, dtype=np.int32),
+                      np.ascontiguousarray(level.xbase, dtype=np.float64),
+                      np.ascontiguousarray(level.targets if targets is None else targets, dtype=np.int32)]
+        nb, kd, xb, tg = self._keep
+        lv = _abi.Level()
+        lv.nslots = int(kd.shape[0])
+        lv.neighbors = _abi.iptr(nb)
+        lv.kind = _abi.iptr(kd)
+        lv.xbase = _abi.dptr(xb)
+        lv.dx = float(level.dx)
+        lv.ntargets = int(tg.size)
+        lv.targets = _abi.iptr(tg)
+        lv.type = type_
+        lv.precision = precision
+        lv.mixed_far_d2 = mixed_far_d2
+        lv.flags = flags
+        lv.d_M, lv.d_com, lv.d_mon, lv.d_L, lv.d_L_c = d_M, d_com, d_mon, d_L, d_L_c
+        self._h = ctypes.c_void_p()
+        ctx._check(lib().octofmm_level_plan_create(ctx._h, ctypes.byref(lv), ctypes.byref(self._h)))
+
+    def execute(self, stream=0):
+        self.ctx._check(lib().octofmm_level_plan_execute(self._h, ctypes.c_void_p(stream)))
+
+    def stats(self):
+        s = _abi.PlanStats()
+        self.ctx._check(lib().octofmm_level_plan_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _abi.PlanStats._fields_}
+
+    def time_stages(self, stream=0, iters=10):
+        ms = (ctypes.c_double * 3)()
+        self.ctx._check(lib().octofmm_level_plan_time_stages(self._h, ctypes.c_void_p(stream), iters, ms))
+        return list(ms)
+
+    def close(self):
+        if self._h:
+            lib().octofmm_level_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
