@@ -44,39 +44,31 @@ def main():
               "in this build", file=sys.stderr)
         sys.exit(2)
     import numpy as np
-    import torch
 
-    from octotiger_amd import _abi, native, workloads
+    from octotiger_amd import _abi, hip, native, workloads
 
-    if not torch.cuda.is_available():
+    if hip.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(3)
     lv = workloads.config2() if args.config == "c2" else workloads.config4()
-    dev = torch.device("cuda:0")
-    M = torch.from_numpy(lv.M).to(dev)
-    com = torch.from_numpy(lv.com).to(dev)
-    mon = torch.from_numpy(lv.mon).to(dev)
-    L = torch.zeros((lv.nslots, 512, 20), dtype=torch.float64, device=dev)
-    Lc = torch.zeros((lv.nslots, 512, 3), dtype=torch.float64, device=dev)
     ctx = native.Context(theta=0.5, device=0)
-    plan = ctx.level_plan(lv, M.data_ptr(), com.data_ptr(), mon.data_ptr(), L.data_ptr(), Lc.data_ptr(),
-                          type_=_abi.RHO, flags=_abi.PLAN_USE_GRAPH)
+    dl = hip.LevelOnDevice(lv)
+    plan = dl.plan(ctx, type_=_abi.RHO, flags=_abi.PLAN_USE_GRAPH)
     stats = plan.stats()
-    stream = torch.cuda.current_stream()
-    sh = stream.cuda_stream
+    stream = hip.Stream()
+    sh = stream.handle
     for _ in range(args.warmup):
         plan.execute(sh)
-    torch.cuda.synchronize()
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
     t0 = time.perf_counter()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(args.steps):
         plan.execute(sh)
     e1.record(stream)
-    torch.cuda.synchronize()
+    e1.synchronize()
     wall = time.perf_counter() - t0
-    ms_step = e0.elapsed_time(e1) / args.steps
+    ms_step = e0.elapsed_ms(e1) / args.steps
     pairs = stats["far_pairs"] + stats["near_pairs"]
     value = pairs / (ms_step * 1e-3)
 
@@ -97,13 +89,12 @@ def main():
         import oracle
         # the accumulated L is (warmup + steps + stage timing) times one pass;
         # one fresh pass on a few targets is compared with the oracle
-        L.zero_()
-        Lc.zero_()
+        dl.zero_outputs()
         plan.execute(sh)
-        torch.cuda.synchronize()
+        stream.synchronize()
         sample = lv.targets[:: max(1, len(lv.targets) // 8)][:8]
         Lr, Lcr = oracle.level_interactions(lv, targets=sample)
-        Lg, Lcg = L.cpu().numpy(), Lc.cpu().numpy()
+        Lg, Lcg = dl.L.numpy(), dl.L_c.numpy()
         err = 0.0
         for a, b in ((Lg[sample], Lr[sample]), (Lcg[sample], Lcr[sample])):
             a2, b2 = a.reshape(-1, a.shape[-1]), b.reshape(-1, b.shape[-1])
@@ -163,6 +154,8 @@ def main():
     }
     print(json.dumps(out), flush=True)
     plan.close()
+    dl.free()
+    stream.destroy()
     ctx.close()
 
 

@@ -13,7 +13,7 @@ import pytest
 import fmm_cases
 import oracle
 from metrics import force_torque_error, normwise
-from octotiger_amd import _abi, interface, native, workloads
+from octotiger_amd import _abi, hip, interface, native, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -81,32 +81,21 @@ def test_interface_mirror_matches_oracle(ctx):
     assert normwise(grid.L, Lr) <= TOL and normwise(grid.L_c, Lcr) <= TOL
 
 
-def _torch():
-    import torch
-    assert torch.cuda.is_available(), "GPU tests need a HIP device"
-    return torch
-
-
 def run_level(ctx, lv, t=_abi.RHO, flags=_abi.PLAN_USE_GRAPH, repeats=1):
-    torch = _torch()
-    dev = torch.device("cuda:0")
-    M = torch.from_numpy(lv.M).to(dev)
-    com = torch.from_numpy(lv.com).to(dev)
-    mon = torch.from_numpy(lv.mon).to(dev)
+    assert hip.device_count() > 0, "GPU tests need a HIP device"
+    dl = hip.LevelOnDevice(lv)
+    stream = hip.Stream()
+    plan = dl.plan(ctx, type_=t, flags=flags)
     outs = []
-    L = torch.zeros((lv.nslots, 512, 20), dtype=torch.float64, device=dev)
-    Lc = torch.zeros((lv.nslots, 512, 3), dtype=torch.float64, device=dev)
-    plan = ctx.level_plan(lv, M.data_ptr(), com.data_ptr(), mon.data_ptr(), L.data_ptr(), Lc.data_ptr(),
-                          type_=t, flags=flags)
-    stream = torch.cuda.current_stream().cuda_stream
     for _ in range(repeats):
-        L.zero_()
-        Lc.zero_()
-        plan.execute(stream)
-        torch.cuda.synchronize()
-        outs.append((L.cpu().numpy().copy(), Lc.cpu().numpy().copy()))
+        dl.zero_outputs()
+        plan.execute(stream.handle)
+        stream.synchronize()
+        outs.append((dl.L.numpy(), dl.L_c.numpy()))
     stats = plan.stats()
     plan.close()
+    stream.destroy()
+    dl.free()
     return outs, stats
 
 
