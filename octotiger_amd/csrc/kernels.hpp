@@ -18,74 +18,100 @@ namespace octofmm {
 
 // ---------------------------------------------------------------------------
 // Pair arithmetic
+//
+// Partner records carry the multiplicity factors of the contraction folded
+// into the multipole: m4, m7, m9 are stored halved (factor_half) and the
+// octupole m10..m19 scaled by factor_sixth = {1,3,3,3,6,3,1,3,3,1}/6, so every
+// term of the contraction is one fused multiply-add.  The target's own
+// octupole used by the angular-momentum correction carries the same scaling,
+// which makes the scaled octupole difference directly the correction weight.
 // ---------------------------------------------------------------------------
+__host__ __device__ constexpr double record_scale(int c) {
+    return (c == 4 || c == 7 || c == 9) ? 0.5
+        : (c == 10 || c == 16 || c == 19) ? 1.0 / 6.0
+        : (c == 14) ? 1.0
+        : (c >= 11 && c <= 18) ? 0.5
+        : 1.0;
+}
+
 struct DTensor {
     double D[20];
     double d2, d3;
 };
 
-// Derivatives of -1/r up to order 3 at dX (compute_d_factors).
+// 1/sqrt(x) to full double precision: hardware estimate plus two Newton steps.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    double e = fma(-(hx * y), y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-(hx * y), y, 0.5);
+    return fma(y, e, y);
+}
+
+// Derivatives of -1/r up to order 3 at dX (compute_d_factors), with the
+// reference's floor r^2 >= 1e-20.
 __device__ __forceinline__ void d_tensor(double x, double y, double z, DTensor& t) {
     const double x00 = x * x, x11 = y * y, x22 = z * z;
-    const double r2 = x00 + x11 + x22;
-    const double r2inv = 1.0 / (r2 < 1.0e-20 ? 1.0e-20 : r2);
-    const double d0 = -sqrt(r2inv);
-    const double d1 = -d0 * r2inv;
-    const double d2 = -3.0 * d1 * r2inv;
-    const double d3 = -5.0 * d2 * r2inv;
+    double r2 = fma(z, z, fma(y, y, x00));
+    r2 = r2 < 1.0e-20 ? 1.0e-20 : r2;
+    const double ri = rsqrt_nr(r2);           // 1/r
+    const double r2inv = ri * ri;
+    const double d1 = ri * r2inv;             // 1/r^3
+    const double d2 = (-3.0 * r2inv) * d1;    // -3/r^5
+    const double d3 = (-5.0 * r2inv) * d2;    // 15/r^7
     t.d2 = d2;
     t.d3 = d3;
     double* D = t.D;
-    D[0] = d0;
+    D[0] = -ri;
     D[1] = x * d1;
     D[2] = y * d1;
     D[3] = z * d1;
-    D[4] = d2 * x00 + d1;
+    D[4] = fma(d2, x00, d1);
     D[5] = d2 * (x * y);
     D[6] = d2 * (x * z);
-    D[7] = d2 * x11 + d1;
+    D[7] = fma(d2, x11, d1);
     D[8] = d2 * (y * z);
-    D[9] = d2 * x22 + d1;
+    D[9] = fma(d2, x22, d1);
     const double d2x = d2 * x, d2y = d2 * y, d2z = d2 * z;
-    D[10] = d3 * x00 * x + 3.0 * d2x;
-    D[11] = d3 * x00 * y + d2y;
-    D[12] = d3 * x00 * z + d2z;
-    D[13] = d3 * x * x11 + d2x;
-    D[14] = d3 * x * (y * z);
-    D[15] = d3 * x * x22 + d2x;
-    D[16] = d3 * x11 * y + 3.0 * d2y;
-    D[17] = d3 * x11 * z + d2z;
-    D[18] = d3 * y * x22 + d2y;
-    D[19] = d3 * x22 * z + 3.0 * d2z;
+    const double d3x00 = d3 * x00, d3x11 = d3 * x11, d3x22 = d3 * x22;
+    D[10] = fma(d3x00, x, 3.0 * d2x);
+    D[11] = fma(d3x00, y, d2y);
+    D[12] = fma(d3x00, z, d2z);
+    D[13] = fma(d3x11, x, d2x);
+    D[14] = (d3 * x) * (y * z);
+    D[15] = fma(d3x22, x, d2x);
+    D[16] = fma(d3x11, y, 3.0 * d2y);
+    D[17] = fma(d3x11, z, d2z);
+    D[18] = fma(d3x22, y, d2y);
+    D[19] = fma(d3x22, z, 3.0 * d2z);
 }
 
-// Order-2/3 part of the contraction of a 20-component partner multipole m with
-// D into the local expansion L (compute_interaction_multipole_non_rho).  With
-// NCOMP == 4 only the potential and gradient rows are formed (p2m targets).
+// Order-2/3 part of the contraction (compute_interaction_multipole_non_rho)
+// with factor-scaled partner components m.  NCOMP == 4: potential and
+// gradient rows only (p2m targets).
 template <int NCOMP>
 __device__ __forceinline__ void contract_high(const double* m, const double* D, double* L) {
-    // multiplicity factors / 2 for order 2 and / 6 for order 3
-    L[0] += m[4] * (D[4] * 0.5) + m[5] * D[5] + m[6] * D[6] + m[7] * (D[7] * 0.5) + m[8] * D[8] +
-        m[9] * (D[9] * 0.5);
-    L[1] += m[4] * (D[10] * 0.5) + m[5] * D[11] + m[6] * D[12] + m[7] * (D[13] * 0.5) + m[8] * D[14] +
-        m[9] * (D[15] * 0.5);
-    L[2] += m[4] * (D[11] * 0.5) + m[5] * D[13] + m[6] * D[14] + m[7] * (D[16] * 0.5) + m[8] * D[17] +
-        m[9] * (D[18] * 0.5);
-    L[3] += m[4] * (D[12] * 0.5) + m[5] * D[14] + m[6] * D[15] + m[7] * (D[17] * 0.5) + m[8] * D[18] +
-        m[9] * (D[19] * 0.5);
-    constexpr double s6 = 1.0 / 6.0;
-    L[0] -= m[10] * (D[10] * s6) + m[11] * (D[11] * 0.5) + m[12] * (D[12] * 0.5) + m[13] * (D[13] * 0.5) +
-        m[14] * D[14] + m[15] * (D[15] * 0.5) + m[16] * (D[16] * s6) + m[17] * (D[17] * 0.5) +
-        m[18] * (D[18] * 0.5) + m[19] * (D[19] * s6);
-    if constexpr (NCOMP == 20) {
-        L[4] += m[0] * D[4] - m[1] * D[10] - m[2] * D[11] - m[3] * D[12];
-        L[5] += m[0] * D[5] - m[1] * D[11] - m[2] * D[13] - m[3] * D[14];
-        L[6] += m[0] * D[6] - m[1] * D[12] - m[2] * D[14] - m[3] * D[15];
-        L[7] += m[0] * D[7] - m[1] * D[13] - m[2] * D[16] - m[3] * D[17];
-        L[8] += m[0] * D[8] - m[1] * D[14] - m[2] * D[17] - m[3] * D[18];
-        L[9] += m[0] * D[9] - m[1] * D[15] - m[2] * D[18] - m[3] * D[19];
+    double l0 = L[0], l1 = L[1], l2 = L[2], l3 = L[3];
+    l0 = fma(m[4], D[4], l0);   l1 = fma(m[4], D[10], l1);  l2 = fma(m[4], D[11], l2);  l3 = fma(m[4], D[12], l3);
+    l0 = fma(m[5], D[5], l0);   l1 = fma(m[5], D[11], l1);  l2 = fma(m[5], D[13], l2);  l3 = fma(m[5], D[14], l3);
+    l0 = fma(m[6], D[6], l0);   l1 = fma(m[6], D[12], l1);  l2 = fma(m[6], D[14], l2);  l3 = fma(m[6], D[15], l3);
+    l0 = fma(m[7], D[7], l0);   l1 = fma(m[7], D[13], l1);  l2 = fma(m[7], D[16], l2);  l3 = fma(m[7], D[17], l3);
+    l0 = fma(m[8], D[8], l0);   l1 = fma(m[8], D[14], l1);  l2 = fma(m[8], D[17], l2);  l3 = fma(m[8], D[18], l3);
+    l0 = fma(m[9], D[9], l0);   l1 = fma(m[9], D[15], l1);  l2 = fma(m[9], D[18], l2);  l3 = fma(m[9], D[19], l3);
 #pragma unroll
-        for (int i = 10; i < 20; ++i) L[i] += m[0] * D[i];
+    for (int k = 10; k < 20; ++k) l0 = fma(-m[k], D[k], l0);
+    L[0] = l0; L[1] = l1; L[2] = l2; L[3] = l3;
+    if constexpr (NCOMP == 20) {
+        // L_ab += m0 D_ab - m_c D_abc
+        L[4] = fma(m[0], D[4], L[4]); L[4] = fma(-m[1], D[10], L[4]); L[4] = fma(-m[2], D[11], L[4]); L[4] = fma(-m[3], D[12], L[4]);
+        L[5] = fma(m[0], D[5], L[5]); L[5] = fma(-m[1], D[11], L[5]); L[5] = fma(-m[2], D[13], L[5]); L[5] = fma(-m[3], D[14], L[5]);
+        L[6] = fma(m[0], D[6], L[6]); L[6] = fma(-m[1], D[12], L[6]); L[6] = fma(-m[2], D[14], L[6]); L[6] = fma(-m[3], D[15], L[6]);
+        L[7] = fma(m[0], D[7], L[7]); L[7] = fma(-m[1], D[13], L[7]); L[7] = fma(-m[2], D[16], L[7]); L[7] = fma(-m[3], D[17], L[7]);
+        L[8] = fma(m[0], D[8], L[8]); L[8] = fma(-m[1], D[14], L[8]); L[8] = fma(-m[2], D[17], L[8]); L[8] = fma(-m[3], D[18], L[8]);
+        L[9] = fma(m[0], D[9], L[9]); L[9] = fma(-m[1], D[15], L[9]); L[9] = fma(-m[2], D[18], L[9]); L[9] = fma(-m[3], D[19], L[9]);
+#pragma unroll
+        for (int i = 10; i < 20; ++i) L[i] = fma(m[0], D[i], L[i]);
     }
 }
 
@@ -93,21 +119,24 @@ __device__ __forceinline__ void contract_high(const double* m, const double* D, 
 // reference subtracts the whole first row of the second-order tensor from
 // each gradient component; kept as is.
 __device__ __forceinline__ void dipole_rows(const double* m, const double* D, double* L) {
-    L[0] -= m[1] * D[1] + m[2] * D[2] + m[3] * D[3];
-    L[1] -= m[1] * (D[4] + D[5] + D[6]);
-    L[2] -= m[2] * (D[5] + D[7] + D[8]);
-    L[3] -= m[3] * (D[6] + D[8] + D[9]);
+    L[0] = fma(-m[1], D[1], L[0]);
+    L[0] = fma(-m[2], D[2], L[0]);
+    L[0] = fma(-m[3], D[3], L[0]);
+    L[1] = fma(-m[1], D[4] + D[5] + D[6], L[1]);
+    L[2] = fma(-m[2], D[5] + D[7] + D[8], L[2]);
+    L[3] = fma(-m[3], D[6] + D[8] + D[9], L[3]);
 }
 
 // Angular-momentum correction from the order-4 tensor (compute_interaction_multipole_rho,
 // :224-334; compute_interaction_p2m_rho, monopole_kernel_templates.hpp:69-187).
-// n[k] = octupole difference (M2L) or partner octupole (p2m), k = 0..9.
+// a[k] = factor-scaled octupole difference (M2L) or partner octupole (p2m).
 __device__ __forceinline__ void am_correction(double x, double y, double z, double d2, double d3,
-    const double* n, double* C) {
-    const double x00 = x * x, x11 = y * y, x22 = z * z;
-    const double d3x00 = d3 * x00, d3x11 = d3 * x11, d3x22 = d3 * x22;
-    const double d3x01 = d3 * x * y, d3x02 = d3 * x * z, d3x12 = d3 * y * z;
-    const double U0 = x00 * d3 + 3.0 * d2 + 5.0 * d3x00;
+    const double* a, double* C) {
+    const double d3x = d3 * x, d3y = d3 * y;
+    const double d3x00 = d3x * x, d3x11 = d3y * y, d3x22 = (d3 * z) * z;
+    const double d3x01 = d3x * y, d3x02 = d3x * z, d3x12 = d3y * z;
+    const double t2 = 3.0 * d2;
+    const double U0 = fma(6.0, d3x00, t2);
     const double U1 = 3.0 * d3x01;
     const double U2 = 3.0 * d3x02;
     const double U3 = d2 + d3x11 + d3x00;
@@ -115,27 +144,25 @@ __device__ __forceinline__ void am_correction(double x, double y, double z, doub
     const double U5 = d2 + d3x22 + d3x00;
     const double U7 = d3x02;
     const double U8 = d3x01;
-    const double U10 = x11 * d3 + 3.0 * d2 + 5.0 * d3x11;
+    const double U10 = fma(6.0, d3x11, t2);
     const double U12 = d2 + d3x22 + d3x11;
     const double U13 = 3.0 * d3x12;
-    const double U14 = x22 * d3 + 3.0 * d2 + 5.0 * d3x22;
-    constexpr double s6 = 1.0 / 6.0;
-    // factor_sixth = multiplicity / 6 : {1,3,3,3,6,3,1,3,3,1} / 6
-    const double a0 = n[0] * s6, a1 = n[1] * 0.5, a2 = n[2] * 0.5, a3 = n[3] * 0.5, a4 = n[4],
-                 a5 = n[5] * 0.5, a6 = n[6] * s6, a7 = n[7] * 0.5, a8 = n[8] * 0.5, a9 = n[9] * s6;
-    C[0] -= a0 * U0 + a1 * U1 + a2 * U2 + a3 * U3 + a4 * U4 + a5 * U5 + a6 * U1 + a7 * U7 + a8 * U8 +
-        a9 * U2;
-    C[1] -= a0 * U1 + a1 * U3 + a2 * U4 + a3 * U1 + a4 * U7 + a5 * U8 + a6 * U10 + a7 * U13 + a8 * U12 +
-        a9 * U13;
-    C[2] -= a0 * U2 + a1 * U4 + a2 * U5 + a3 * U7 + a4 * U8 + a5 * U2 + a6 * U13 + a7 * U12 + a8 * U13 +
-        a9 * U14;
+    const double U14 = fma(6.0, d3x22, t2);
+    double c0 = C[0], c1 = C[1], c2 = C[2];
+    c0 = fma(-a[0], U0, c0);  c1 = fma(-a[0], U1, c1);  c2 = fma(-a[0], U2, c2);
+    c0 = fma(-a[1], U1, c0);  c1 = fma(-a[1], U3, c1);  c2 = fma(-a[1], U4, c2);
+    c0 = fma(-a[2], U2, c0);  c1 = fma(-a[2], U4, c1);  c2 = fma(-a[2], U5, c2);
+    c0 = fma(-a[3], U3, c0);  c1 = fma(-a[3], U1, c1);  c2 = fma(-a[3], U7, c2);
+    c0 = fma(-a[4], U4, c0);  c1 = fma(-a[4], U7, c1);  c2 = fma(-a[4], U8, c2);
+    c0 = fma(-a[5], U5, c0);  c1 = fma(-a[5], U8, c1);  c2 = fma(-a[5], U2, c2);
+    c0 = fma(-a[6], U1, c0);  c1 = fma(-a[6], U10, c1); c2 = fma(-a[6], U13, c2);
+    c0 = fma(-a[7], U7, c0);  c1 = fma(-a[7], U13, c1); c2 = fma(-a[7], U12, c2);
+    c0 = fma(-a[8], U8, c0);  c1 = fma(-a[8], U12, c1); c2 = fma(-a[8], U13, c2);
+    c0 = fma(-a[9], U2, c0);  c1 = fma(-a[9], U13, c1); c2 = fma(-a[9], U14, c2);
+    C[0] = c0; C[1] = c1; C[2] = c2;
 }
 
-// One far M2L pair: partner record p (m0, M1..M19, Y, mon) onto target X.
-template <bool RHO>
-__device__ __forceinline__ void m2l_far(const double* __restrict__ p, double X0, double X1, double X2,
-    double mc0, const double* mc, double* L, double* C) {
-    double m[20];
+__device__ __forceinline__ void load_record20(const double* __restrict__ p, double* m) {
     const double2* p2 = reinterpret_cast<const double2*>(p);
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
@@ -143,40 +170,47 @@ __device__ __forceinline__ void m2l_far(const double* __restrict__ p, double X0,
         m[2 * i] = v.x;
         m[2 * i + 1] = v.y;
     }
-    const double2 yv = p2[10];
+}
+
+// One far M2L pair: partner record p (m0, scaled M1..M19, Y, mon) onto the
+// target at X with mass reciprocal inv_mc0 and scaled octupole mc[0..9].
+template <bool RHO>
+__device__ __forceinline__ void m2l_far(const double* __restrict__ p, double X0, double X1, double X2,
+    double inv_mc0, const double* mc, double* L, double* C) {
+    double m[20];
+    load_record20(p, m);
+    const double2 yv = reinterpret_cast<const double2*>(p)[10];
     const double x = X0 - yv.x, y = X1 - yv.y, z = X2 - p[22];
     DTensor t;
     d_tensor(x, y, z, t);
-    L[0] += m[0] * t.D[0];
-    L[1] += m[0] * t.D[1];
-    L[2] += m[0] * t.D[2];
-    L[3] += m[0] * t.D[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) L[i] = fma(m[0], t.D[i], L[i]);
     if constexpr (!RHO) dipole_rows(m, t.D, L);
     contract_high<20>(m, t.D, L);
     if constexpr (RHO) {
-        const double n0c = m[0] / mc0;
-        double n[10];
+        const double n0c = m[0] * inv_mc0;
+        double a[10];
 #pragma unroll
-        for (int k = 0; k < 10; ++k) n[k] = m[10 + k] - mc[k] * n0c;
-        am_correction(x, y, z, t.d2, t.d3, n, C);
+        for (int k = 0; k < 10; ++k) a[k] = fma(-mc[k], n0c, m[10 + k]);
+        am_correction(x, y, z, t.d2, t.d3, a, C);
     }
 }
 
 // One near pair with a monopole-only partner of mass mn at Y.
 template <bool RHO>
 __device__ __forceinline__ void m2l_near(double mn, double Y0, double Y1, double Y2, double X0, double X1,
-    double X2, double mc0, const double* mc, double* L, double* C) {
+    double X2, double inv_mc0, const double* mc, double* L, double* C) {
     const double x = X0 - Y0, y = X1 - Y1, z = X2 - Y2;
     DTensor t;
     d_tensor(x, y, z, t);
 #pragma unroll
-    for (int i = 0; i < 20; ++i) L[i] += mn * t.D[i];
+    for (int i = 0; i < 20; ++i) L[i] = fma(mn, t.D[i], L[i]);
     if constexpr (RHO) {
-        const double n0c = mn / mc0;
-        double n[10];
+        const double n0c = mn * inv_mc0;
+        double a[10];
 #pragma unroll
-        for (int k = 0; k < 10; ++k) n[k] = -(mc[k] * n0c);
-        am_correction(x, y, z, t.d2, t.d3, n, C);
+        for (int k = 0; k < 10; ++k) a[k] = -(mc[k] * n0c);
+        am_correction(x, y, z, t.d2, t.d3, a, C);
     }
 }
 
@@ -185,20 +219,12 @@ template <bool RHO>
 __device__ __forceinline__ void p2m_pair(const double* __restrict__ p, double X0, double X1, double X2,
     double* L, double* C) {
     double m[20];
-    const double2* p2 = reinterpret_cast<const double2*>(p);
-#pragma unroll
-    for (int i = 0; i < 10; ++i) {
-        const double2 v = p2[i];
-        m[2 * i] = v.x;
-        m[2 * i + 1] = v.y;
-    }
+    load_record20(p, m);
     const double x = X0 - p[20], y = X1 - p[21], z = X2 - p[22];
     DTensor t;
     d_tensor(x, y, z, t);
-    L[0] += m[0] * t.D[0];
-    L[1] += m[0] * t.D[1];
-    L[2] += m[0] * t.D[2];
-    L[3] += m[0] * t.D[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) L[i] = fma(m[0], t.D[i], L[i]);
     if constexpr (!RHO) dipole_rows(m, t.D, L);
     contract_high<4>(m, t.D, L);
     if constexpr (RHO) am_correction(x, y, z, t.d2, t.d3, m + 10, C);

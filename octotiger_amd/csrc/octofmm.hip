@@ -82,6 +82,8 @@ __global__ __launch_bounds__(256) void stage_records(StageArgs a) {
 #pragma unroll
         for (int i = 0; i < kRecDoubles; ++i) v[i] = 0.0;
     }
+#pragma unroll
+    for (int i = 4; i < 20; ++i) v[i] *= record_scale(i);
     double2* o2 = reinterpret_cast<double2*>(o);
 #pragma unroll
     for (int i = 0; i < kRecDoubles / 2; ++i) o2[i] = make_double2(v[2 * i], v[2 * i + 1]);
@@ -127,7 +129,7 @@ __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_n
     const Tables* __restrict__ tab = a.tab;
     const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
     const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
-    const double mc0 = trec[0];
+    const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
     double mc[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
@@ -143,9 +145,9 @@ __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_n
         const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
         if (ns < 0) fb = 0;
         const double* pb = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
-#pragma unroll 1
+#pragma unroll
         for (int j = 0; j < 8; ++j) {
-            if (fb & (1u << j)) m2l_far<RHO>(pb + j * kRecDoubles, X0, X1, X2, mc0, mc, L, C);
+            if (fb & (1u << j)) m2l_far<RHO>(pb + j * kRecDoubles, X0, X1, X2, inv_mc0, mc, L, C);
         }
     }
     if (flags & kItemNear) {
@@ -160,7 +162,7 @@ __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_n
             for (int j = 0; j < 8; ++j) {
                 if (nb & (1u << j)) {
                     const double* p = pb + j * kRecDoubles;
-                    m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, mc0, mc, L, C);
+                    m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, inv_mc0, mc, L, C);
                 }
             }
         }
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(512) void root_kernel(RootArgs a) {
     const int tx = t >> 6, ty = (t >> 3) & 7, tz = t & 7;
     const double* trec = a.rec + (size_t)record_of_cell(tx, ty, tz) * kRecDoubles;
     const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
-    const double mc0 = trec[0];
+    const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
     double mc[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(512) void root_kernel(RootArgs a) {
         const int dx = px - tx, dy = py - ty, dz = pz - tz;
         const bool inbox = dx >= -5 && dx <= 5 && dy >= -5 && dy <= 5 && dz >= -5 && dz <= 5;
         if (inbox && a.tab->root_excluded[(dx + 5) * 121 + (dy + 5) * 11 + (dz + 5)]) continue;
-        m2l_far<RHO>(a.rec + (size_t)record_of_cell(px, py, pz) * kRecDoubles, X0, X1, X2, mc0, mc, L, C);
+        m2l_far<RHO>(a.rec + (size_t)record_of_cell(px, py, pz) * kRecDoubles, X0, X1, X2, inv_mc0, mc, L, C);
     }
     double* o = a.partial + ((size_t)blockIdx.x * kCells + t) * kRecDoubles;
 #pragma unroll
