@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--config", default="c2", choices=["c2", "c4"])
     ap.add_argument("--cpu-sample", type=int, default=64, help="target sub-grids timed on the CPU (0 = skip)")
     ap.add_argument("--no-check", action="store_true", help="skip the post-run parity spot check")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run 3 device-to-device copies of this many MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -53,8 +55,17 @@ def main():
     lv = workloads.config2() if args.config == "c2" else workloads.config4()
     ctx = native.Context(theta=0.5, device=0)
     dl = hip.LevelOnDevice(lv)
-    plan = dl.plan(ctx, type_=_abi.RHO, flags=_abi.PLAN_USE_GRAPH)
+    plan = dl.plan(ctx, type_=_abi.RHO, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
     stats = plan.stats()
+    if args.copy_calibration_mib > 0:
+        nb = args.copy_calibration_mib << 20
+        src, dst = hip.DeviceArray((nb // 8,)), hip.DeviceArray((nb // 8,))
+        src.zero()
+        for _ in range(3):
+            hip.check(hip.rt().hipMemcpy(dst.ptr, src.ptr, nb, hip.D2D), "hipMemcpy D2D")
+        hip.synchronize()
+        src.free()
+        dst.free()
     stream = hip.Stream()
     sh = stream.handle
     for _ in range(args.warmup):
@@ -136,6 +147,7 @@ def main():
             "far_pairs_per_pass": int(stats["far_pairs"]),
             "flops_per_pair": FLOPS_PER_PAIR["rho"],
             "precision": "fp64",
+            "workgroup_waves": int(stats["variant"]),
         },
         "roofline": {
             "bound": "valu",

@@ -172,8 +172,44 @@ __device__ __forceinline__ void load_record20(const double* __restrict__ p, doub
     }
 }
 
-// One far M2L pair: partner record p (m0, scaled M1..M19, Y, mon) onto the
+// A partner record held in registers (24 doubles as 12 double2).
+struct Rec {
+    double2 v[12];
+};
+
+__device__ __forceinline__ void load_rec(const double* __restrict__ p, Rec& r) {
+    const double2* p2 = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) r.v[i] = p2[i];
+}
+
+// One far M2L pair: partner record (m0, scaled M1..M19, Y, mon) onto the
 // target at X with mass reciprocal inv_mc0 and scaled octupole mc[0..9].
+template <bool RHO>
+__device__ __forceinline__ void m2l_far_rec(const Rec& r, double X0, double X1, double X2, double inv_mc0,
+    const double* mc, double* L, double* C) {
+    double m[20];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        m[2 * i] = r.v[i].x;
+        m[2 * i + 1] = r.v[i].y;
+    }
+    const double x = X0 - r.v[10].x, y = X1 - r.v[10].y, z = X2 - r.v[11].x;
+    DTensor t;
+    d_tensor(x, y, z, t);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) L[i] = fma(m[0], t.D[i], L[i]);
+    if constexpr (!RHO) dipole_rows(m, t.D, L);
+    contract_high<20>(m, t.D, L);
+    if constexpr (RHO) {
+        const double n0c = m[0] * inv_mc0;
+        double a[10];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) a[k] = fma(-mc[k], n0c, m[10 + k]);
+        am_correction(x, y, z, t.d2, t.d3, a, C);
+    }
+}
+
 template <bool RHO>
 __device__ __forceinline__ void m2l_far(const double* __restrict__ p, double X0, double X1, double X2,
     double inv_mc0, const double* mc, double* L, double* C) {

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -96,9 +97,10 @@ struct LaneGeom {
     int rec;          // record index inside the slot
 };
 
-__device__ __forceinline__ LaneGeom lane_geom() {
+// wave w (0..7 across the sub-grid) owns coarse block ((w>>2)&1, (w>>1)&1, w&1)
+__device__ __forceinline__ LaneGeom lane_geom(int w) {
     LaneGeom g;
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int l = threadIdx.x & 63;
     const int c = l >> 3;
     g.k = l & 7;
     g.Cx = ((w >> 2) & 1) * 2 + ((c >> 2) & 1);
@@ -124,8 +126,8 @@ __device__ __forceinline__ int partner_base(const LaneGeom& g, int ax, int ay, i
 }
 
 template <bool RHO>
-__device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind) {
-    const LaneGeom g = lane_geom();
+__device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind, int w) {
+    const LaneGeom g = lane_geom(w);
     const Tables* __restrict__ tab = a.tab;
     const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
     const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
@@ -138,17 +140,36 @@ __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_n
     for (int i = 0; i < 20; ++i) L[i] = 0.0;
 
     const int nd = tab->ndelta;
-    for (int di = 0; di < nd; ++di) {
+    // far pass, software-pipelined: the record of the next pair is loaded
+    // (by every lane, from a valid address) while the current one is computed
+    auto far_base = [&](int di, unsigned& fb) -> const double* {
         const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
-        unsigned fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
+        fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
         int ns, raw;
         const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
         if (ns < 0) fb = 0;
-        const double* pb = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
+        return a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
+    };
+    unsigned fb;
+    const double* pb = far_base(0, fb);
+    Rec nxt;
+    load_rec(pb, nxt);
+    for (int di = 0; di < nd; ++di) {
+        unsigned fbn = 0;
+        const double* pbn = pb;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            if (fb & (1u << j)) m2l_far<RHO>(pb + j * kRecDoubles, X0, X1, X2, inv_mc0, mc, L, C);
+            const Rec cur = nxt;
+            if (j < 7) {
+                load_rec(pb + (j + 1) * kRecDoubles, nxt);
+            } else if (di + 1 < nd) {
+                pbn = far_base(di + 1, fbn);
+                load_rec(pbn, nxt);
+            }
+            if (fb & (1u << j)) m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
         }
+        pb = pbn;
+        fb = fbn;
     }
     if (flags & kItemNear) {
         for (int di = 0; di < nd; ++di) {
@@ -178,9 +199,208 @@ __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_n
     }
 }
 
+// Waits for this wave's LDS writes (lgkmcnt(0), global loads left in flight)
+// and keeps the compiler from moving LDS accesses across the point.  The
+// lanes of a wave execute each LDS instruction together and in order, so the
+// written records are then visible to every lane of the wave.
+__device__ __forceinline__ void lds_wave_sync() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // vmcnt = 63, expcnt = 7, lgkmcnt = 0
+    __builtin_amdgcn_wave_barrier();
+}
+
+// LDS stage of one wave: 8 coarse-cell groups of 8 partner records
+// (12 double2 each), each group padded by one double2 so the 8 broadcast
+// addresses of a read fall into different banks.
+constexpr int kStageGroup = 8 * 12 + 1;
+constexpr int kStageWave = 8 * kStageGroup;
+
+// Mapping 2: mapping 0 with the partner records of each coarse offset staged
+// once per wave in LDS and read back as 8-lane broadcasts by the 8 child
+// iterations.  Staging is fully coalesced: the 8 partner coarse cells of the
+// wave each hold their 8 records contiguously (1536 B), and lane l moves
+// 16-byte chunk l of every 1 KiB of that data.
 template <bool RHO>
-__device__ void leaf_item(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind) {
-    const LaneGeom g = lane_geom();
+__device__ void m2l_item_lds(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind, int w,
+    double2* __restrict__ stage) {
+    const LaneGeom g = lane_geom(w);
+    const int l = threadIdx.x & 63, c = l >> 3;
+    const Tables* __restrict__ tab = a.tab;
+    const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
+    const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
+    const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
+    double mc[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
+    double L[20], C[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 20; ++i) L[i] = 0.0;
+    const int nd = tab->ndelta;
+    // wave-uniform: the wave's coarse block and the neighbour table row
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const int bx = ((wu >> 2) & 1) * 2, by = ((wu >> 1) & 1) * 2, bz = (wu & 1) * 2;
+    const int32_t* nbr_row = a.nbr + (size_t)__builtin_amdgcn_readfirstlane(slot) * 27;
+    // base address of the records of partner coarse cell (block + group g) + delta
+    auto group_base = [&](int gi, int ax, int ay, int az) -> const double2* {
+        const int px = bx + ((gi >> 2) & 1) + ax, py = by + ((gi >> 1) & 1) + ay, pz = bz + (gi & 1) + az;
+        const int raw = (px < 0 ? 0 : (px < 4 ? 1 : 2)) + 3 * (py < 0 ? 0 : (py < 4 ? 1 : 2)) +
+            9 * (pz < 0 ? 0 : (pz < 4 ? 1 : 2));
+        const int ns = nbr_row[raw];
+        const int cb = ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
+        return reinterpret_cast<const double2*>(a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles);
+    };
+    // chunk i*64 + l of the 768 staged chunks: group q / 96, offset q % 96
+    double2 pre[12];
+    auto fetch = [&](int di) {
+        const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+        const double2* gb[8];
+#pragma unroll
+        for (int gi = 0; gi < 8; ++gi) gb[gi] = group_base(gi, ax, ay, az);
+#pragma unroll
+        for (int i = 0; i < 12; ++i) {
+            const int q = i * 64 + l;
+            const int glo = (i * 64) / 96, ghi = (i * 64 + 63) / 96;
+            const int gq = q >= (glo + 1) * 96 ? ghi : glo;
+            pre[i] = (gq == glo ? gb[glo] : gb[ghi])[q - gq * 96];
+        }
+    };
+    auto far_bits = [&](int di) -> unsigned {
+        unsigned fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
+        int ns, raw;
+        partner_base(g, tab->delta[di][0], tab->delta[di][1], tab->delta[di][2], s_nbr, ns, raw);
+        return ns < 0 ? 0u : fb;
+    };
+    const double2* group = stage + c * kStageGroup;
+    fetch(0);
+    for (int di = 0; di < nd; ++di) {
+#pragma unroll
+        for (int i = 0; i < 12; ++i) {
+            const int q = i * 64 + l;
+            const int gq = q / 96;
+            stage[gq * kStageGroup + (q - gq * 96)] = pre[i];
+        }
+        lds_wave_sync();
+        const unsigned fb = far_bits(di);
+        if (di + 1 < nd) fetch(di + 1);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (fb & (1u << j)) {
+                Rec cur;
+#pragma unroll
+                for (int i = 0; i < 12; ++i) cur.v[i] = group[j * 12 + i];
+                m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
+            }
+        }
+        lds_wave_sync();
+    }
+    if (flags & kItemNear) {
+        for (int di = 0; di < nd; ++di) {
+            const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+            unsigned nb = (unsigned)(tab->nearbits[di] >> (8 * g.k)) & 0xffu;
+            int ns, raw;
+            const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
+            if (ns < 0 || s_kind[raw] != OCTOFMM_SLOT_LEAF) nb = 0;
+            const double* pb2 = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
+#pragma unroll 1
+            for (int j = 0; j < 8; ++j) {
+                if (nb & (1u << j)) {
+                    const double* p = pb2 + j * kRecDoubles;
+                    m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, inv_mc0, mc, L, C);
+                }
+            }
+        }
+    }
+    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
+#pragma unroll
+    for (int i = 0; i < 20; ++i) Lo[i] += L[i];
+    if (RHO) {
+        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
+        Co[0] = C[0];
+        Co[1] = C[1];
+        Co[2] = C[2];
+    }
+}
+
+// Mapping 1: wave = child class k over the 64 coarse cells of the sub-grid;
+// the (delta, j) partner list is the same for every lane.
+template <bool RHO>
+__device__ void m2l_item_classwave(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind,
+    int k) {
+    const int l = threadIdx.x & 63;
+    LaneGeom g;
+    g.k = k;
+    g.Cx = l >> 4;
+    g.Cy = (l >> 2) & 3;
+    g.Cz = l & 3;
+    const int fx = 2 * g.Cx + ((k >> 2) & 1), fy = 2 * g.Cy + ((k >> 1) & 1), fz = 2 * g.Cz + (k & 1);
+    g.cell = fx * 64 + fy * 8 + fz;
+    g.rec = (g.Cx * 16 + g.Cy * 4 + g.Cz) * 8 + k;
+    const Tables* __restrict__ tab = a.tab;
+    const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
+    const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
+    const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
+    double mc[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
+    double L[20], C[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 20; ++i) L[i] = 0.0;
+    auto base_of = [&](int di, bool& ok, int& raw) -> const double* {
+        int ns;
+        const int cb = partner_base(g, tab->delta[di][0], tab->delta[di][1], tab->delta[di][2], s_nbr, ns, raw);
+        ok = ns >= 0;
+        return a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
+    };
+    {
+        const uint16_t* list = tab->farlist[k];
+        const int n = tab->nfar_k[k];
+        int e = list[0], di = e >> 3, raw;
+        bool okn;
+        const double* bnx = base_of(di, okn, raw);
+        Rec nxt;
+        load_rec(bnx + (e & 7) * kRecDoubles, nxt);
+#pragma unroll 2
+        for (int q = 0; q < n; ++q) {
+            const Rec cur = nxt;
+            const bool okc = okn;
+            if (q + 1 < n) {
+                const int e1 = list[q + 1], di1 = e1 >> 3;
+                if (di1 != di) {
+                    di = di1;
+                    bnx = base_of(di1, okn, raw);
+                }
+                load_rec(bnx + (e1 & 7) * kRecDoubles, nxt);
+            }
+            if (okc) m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
+        }
+    }
+    if (flags & kItemNear) {
+        const uint16_t* list = tab->nearlist[k];
+        const int n = tab->nnear_k[k];
+        for (int q = 0; q < n; ++q) {
+            const int e = list[q];
+            bool ok;
+            int raw;
+            const double* pb = base_of(e >> 3, ok, raw);
+            if (ok && s_kind[raw] == OCTOFMM_SLOT_LEAF) {
+                const double* p = pb + (e & 7) * kRecDoubles;
+                m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, inv_mc0, mc, L, C);
+            }
+        }
+    }
+    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
+#pragma unroll
+    for (int i = 0; i < 20; ++i) Lo[i] += L[i];
+    if (RHO) {
+        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
+        Co[0] = C[0];
+        Co[1] = C[1];
+        Co[2] = C[2];
+    }
+}
+
+template <bool RHO>
+__device__ void leaf_item(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind, int w) {
+    const LaneGeom g = lane_geom(w);
     const Tables* __restrict__ tab = a.tab;
     const double* xc = a.com + ((size_t)slot * kCells + g.cell) * 3;
     const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
@@ -226,11 +446,18 @@ __device__ void leaf_item(const LevelArgs& a, int slot, int flags, const int* s_
     }
 }
 
-template <bool RHO>
-__global__ __launch_bounds__(512) void level_kernel(LevelArgs a) {
+// One target sub-grid is split over 8 / WAVES workgroups of WAVES waves.  The
+// workgroup index is first spread over the XCDs (contiguous runs per XCD),
+// so the parts of one sub-grid and its neighbours share an L2.
+template <bool RHO, int WAVES, int MAPPING>
+__global__ __launch_bounds__(64 * WAVES) void level_kernel(LevelArgs a) {
+    constexpr int kParts = 8 / WAVES;
     __shared__ int s_nbr[27];
     __shared__ int s_kind[27];
-    const int item = xcd_item(blockIdx.x, a.nitems);
+    __shared__ double2 s_stage[MAPPING == 2 ? WAVES * kStageWave : 1];
+    const int v = xcd_item(blockIdx.x, gridDim.x);
+    const int item = v / kParts;
+    const int w = (v % kParts) * WAVES + (threadIdx.x >> 6);
     const int2 it = a.items[item];
     if (threadIdx.x < 27) {
         const int n = a.nbr[(size_t)it.x * 27 + threadIdx.x];
@@ -239,9 +466,31 @@ __global__ __launch_bounds__(512) void level_kernel(LevelArgs a) {
     }
     __syncthreads();
     if (it.y & kItemLeaf)
-        leaf_item<RHO>(a, it.x, it.y, s_nbr, s_kind);
+        leaf_item<RHO>(a, it.x, it.y, s_nbr, s_kind, w);
+    else if (MAPPING == 1)
+        m2l_item_classwave<RHO>(a, it.x, it.y, s_nbr, s_kind, w);
+    else if (MAPPING == 2)
+        m2l_item_lds<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage + (threadIdx.x >> 6) * kStageWave);
     else
-        m2l_item<RHO>(a, it.x, it.y, s_nbr, s_kind);
+        m2l_item<RHO>(a, it.x, it.y, s_nbr, s_kind, w);
+}
+
+template <bool RHO>
+void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la) {
+    const int waves = variant % 100;
+    const int parts = 8 / waves;
+    const dim3 grid(nitems * parts), block(64 * waves);
+    switch (variant) {
+    case 8: hipLaunchKernelGGL((level_kernel<RHO, 8, 0>), grid, block, 0, st, la); break;
+    case 4: hipLaunchKernelGGL((level_kernel<RHO, 4, 0>), grid, block, 0, st, la); break;
+    case 2: hipLaunchKernelGGL((level_kernel<RHO, 2, 0>), grid, block, 0, st, la); break;
+    case 108: hipLaunchKernelGGL((level_kernel<RHO, 8, 1>), grid, block, 0, st, la); break;
+    case 104: hipLaunchKernelGGL((level_kernel<RHO, 4, 1>), grid, block, 0, st, la); break;
+    case 102: hipLaunchKernelGGL((level_kernel<RHO, 2, 1>), grid, block, 0, st, la); break;
+    case 208: hipLaunchKernelGGL((level_kernel<RHO, 8, 2>), grid, block, 0, st, la); break;
+    case 204: hipLaunchKernelGGL((level_kernel<RHO, 4, 2>), grid, block, 0, st, la); break;
+    default: hipLaunchKernelGGL((level_kernel<RHO, 2, 2>), grid, block, 0, st, la); break;
+    }
 }
 
 // Root sub-grid: every pair of its 512 cells except self and near offsets
@@ -348,6 +597,7 @@ struct octofmm_plan {
     double* d_fxbase = nullptr;
     int2* d_items = nullptr;
     octofmm_plan_stats stats;
+    int variant = 8;           // level-kernel variant (see kVariants)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 };
@@ -570,9 +820,9 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         la.inv_dx = 1.0 / dx;
         la.neg_inv_dx = -1.0 / dx;
         if (rho)
-            hipLaunchKernelGGL(level_kernel<true>, dim3(1), dim3(512), 0, st, la);
+            launch_level<true>(8, 1, st, la);
         else
-            hipLaunchKernelGGL(level_kernel<false>, dim3(1), dim3(512), 0, st, la);
+            launch_level<false>(8, 1, st, la);
         HIPC(c, hipGetLastError());
     }
     // results: L (20) then L_c (3) per cell into h_out
@@ -759,11 +1009,72 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         la.inv_dx = 1.0 / p->dx;
         la.neg_inv_dx = -1.0 / p->dx;
         if (p->type == OCTOFMM_RHO)
-            hipLaunchKernelGGL(level_kernel<true>, dim3(p->nitems), dim3(512), 0, st, la);
+            launch_level<true>(p->variant, p->nitems, st, la);
         else
-            hipLaunchKernelGGL(level_kernel<false>, dim3(p->nitems), dim3(512), 0, st, la);
+            launch_level<false>(p->variant, p->nitems, st, la);
         HIPC(c, hipGetLastError());
     }
+    return OCTOFMM_OK;
+}
+
+// Times each workgroup shape on the live level and keeps the fastest.  The
+// trial passes write into private scratch outputs, never the caller's L / L_c.
+static octofmm_status plan_autotune_run(octofmm_plan* p);
+
+static octofmm_status plan_autotune(octofmm_plan* p) {
+    octofmm_context* c = p->ctx;
+    double* userL = p->d_L;
+    double* userLc = p->d_Lc;
+    double *tL = nullptr, *tLc = nullptr;
+    const size_t n = (size_t)p->nslots * kCells;
+    if (hipMalloc((void**)&tL, n * 20 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&tLc, n * 3 * sizeof(double)) != hipSuccess) {
+        (void)hipFree(tL);
+        return fail(c, OCTOFMM_ERR_ALLOCATION, "autotune scratch allocation failed");
+    }
+    p->d_L = tL;
+    p->d_Lc = tLc;
+    const octofmm_status s = plan_autotune_run(p);
+    (void)hipStreamSynchronize(c->stream);
+    p->d_L = userL;
+    p->d_Lc = userLc;
+    (void)hipFree(tL);
+    (void)hipFree(tLc);
+    return s;
+}
+
+static octofmm_status plan_autotune_run(octofmm_plan* p) {
+    octofmm_context* c = p->ctx;
+    hipStream_t st = c->stream;
+    HIPC(c, hipStreamSynchronize(st));
+    octofmm_status s = plan_launch(p, st, 1);
+    if (s != OCTOFMM_OK) return s;
+    hipEvent_t e0, e1;
+    HIPC(c, hipEventCreate(&e0));
+    HIPC(c, hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_w = 8;
+    for (int w : kVariants) {
+        p->variant = w;
+        s = plan_launch(p, st, 2);   // warm-up
+        if (s != OCTOFMM_OK) return s;
+        HIPC(c, hipEventRecord(e0, st));
+        for (int i = 0; i < 3; ++i) {
+            s = plan_launch(p, st, 2);
+            if (s != OCTOFMM_OK) return s;
+        }
+        HIPC(c, hipEventRecord(e1, st));
+        HIPC(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPC(c, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_w = w;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->variant = best_w;
     return OCTOFMM_OK;
 }
 
@@ -886,6 +1197,25 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
         (!items.empty() &&
             hipMemcpy(p->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess))
         return cleanup(OCTOFMM_ERR_HIP, "plan table upload failed");
+    const char* forced = std::getenv("OCTOFMM_VARIANT");
+    bool forced_ok = false;
+    if (forced)
+        for (int v : kVariants)
+            if (std::atoi(forced) == v) {
+                p->variant = v;
+                forced_ok = true;
+            }
+    if (forced_ok) {
+    } else if ((lv->flags & OCTOFMM_PLAN_AUTOTUNE) && p->nitems > 0) {
+        // time each workgroup shape on the live level (outputs are scratch
+        // here: the caller zeroes L before a real pass) and keep the fastest
+        octofmm_status ts = plan_autotune(p);
+        if (ts != OCTOFMM_OK) {
+            plan_free(p);
+            return ts;
+        }
+    }
+    p->stats.variant = p->variant;
     if (lv->flags & OCTOFMM_PLAN_USE_GRAPH) {
         hipStream_t cs = c->stream;
         if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)

@@ -42,7 +42,22 @@ struct Tables {
     uint8_t root_excluded[kStencilBox];   // root kernel: near offsets and self
     uint8_t phase[kStencilBox];       // 1 far, 0 near (inside the stencil), 1 elsewhere
     uint8_t mask[kStencilBox];        // union stencil mask
+    // per child class k: the (delta, j) pairs of its far and near partners,
+    // packed as delta_index * 8 + j, ordered by delta
+    int32_t nfar_k[8];
+    int32_t nnear_k[8];
+    uint16_t farlist[8][kMaxDelta * 8];
+    uint16_t nearlist[8][kMaxDelta * 8];
 };
+
+// Level-kernel variants: lane mapping x 100 + waves per workgroup.
+//   mapping 0: a wave owns a 2x2x2 block of coarse cells, lane = (coarse cell,
+//              child); the 8 lanes of a coarse cell share every partner load.
+//   mapping 1: a wave owns one child class over all 64 coarse cells; its
+//              partner list is uniform, so no lane idles on near pairs.
+//   mapping 2: mapping 0 with each coarse offset's partner records staged
+//              once per wave in LDS and read back as broadcasts.
+constexpr int kVariants[9] = {8, 4, 2, 108, 104, 102, 208, 204, 202};
 
 // Builds the tables; returns false when theta is below the floor or the
 // stencil fails the reference's symmetry / width checks.

@@ -138,6 +138,16 @@ bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
     }
     t.nfar_per_cell = nf;
     t.nnear_per_cell = nn;
+    for (int k = 0; k < 8; ++k) {
+        int a = 0, b = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < 8; ++j) {
+                if ((t.farbits[i] >> (8 * k + j)) & 1) t.farlist[k][a++] = (uint16_t)(i * 8 + j);
+                if ((t.nearbits[i] >> (8 * k + j)) & 1) t.nearlist[k][b++] = (uint16_t)(i * 8 + j);
+            }
+        t.nfar_k[k] = a;
+        t.nnear_k[k] = b;
+    }
     return true;
 }
 

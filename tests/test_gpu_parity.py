@@ -147,3 +147,35 @@ def test_level_config2_full_size(ctx):
     assert normwise(L[tg], Lr[tg]) <= TOL and normwise(Lc[tg], Lcr[tg]) <= TOL
     ef, et = force_torque_error(L[tg], Lc[tg], Lr[tg], Lcr[tg], lv.M[tg, :, 0], lv.com[tg])
     assert ef <= TOL and et <= TOL
+
+
+@pytest.mark.parametrize("variant", [8, 4, 2, 108, 104, 102, 208, 204, 202])
+def test_level_kernel_variants(ctx, variant, monkeypatch):
+    """Every lane mapping and workgroup shape of the level kernel gives the
+    oracle's result (mixed level: M2L far + near passes, p2p, p2m)."""
+    monkeypatch.setenv("OCTOFMM_VARIANT", str(variant))
+    lv = workloads.mixed_level((6, 6, 6), seed=13, leaf_fraction=0.5)
+    print(f"variant {variant}", flush=True)
+    outs, stats = run_level(ctx, lv, _abi.RHO)
+    assert stats["variant"] == variant
+    L, Lc = outs[0]
+    Lr, Lcr = oracle.level_interactions(lv)
+    tg = lv.targets
+    assert normwise(L[tg], Lr[tg]) <= TOL and normwise(Lc[tg], Lcr[tg]) <= TOL
+
+
+def test_autotuned_plan_leaves_outputs_untouched_until_executed(ctx):
+    lv = workloads.uniform_level((4, 4, 4), ((1, 3),) * 3, seed=21)
+    dl = hip.LevelOnDevice(lv)
+    plan = dl.plan(ctx, type_=_abi.RHO, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
+    assert plan.stats()["variant"] in (8, 4, 2)
+    assert not dl.L.numpy().any() and not dl.L_c.numpy().any()
+    stream = hip.Stream()
+    plan.execute(stream.handle)
+    stream.synchronize()
+    Lr, Lcr = oracle.level_interactions(lv)
+    tg = lv.targets
+    assert normwise(dl.L.numpy()[tg], Lr[tg]) <= TOL
+    plan.close()
+    stream.destroy()
+    dl.free()
