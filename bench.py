@@ -3,14 +3,17 @@
 One step = one pass of the level interaction path over the batched uniform
 level of 512 non-leaf 8^3 sub-grids (80^3 cells, theta = 0.5, RHO with the
 angular-momentum correction): AoS -> partner-record staging plus the
-interaction kernel, replayed as one captured hipGraph.  Inputs are resident in
-HBM before timing starts.
+interaction kernel, replayed as one captured hipGraph, kernel variant chosen
+by the plan's autotuner.  Inputs are resident in HBM before timing starts.
 
-Prints one JSON line: cell-interactions/s (far M2L pairs per second), the
-roofline of the dominant kernel (FP64 VALU), and a CPU baseline (the FP64
-oracle on the host cores, on a bounded sample of the same level).
+Prints one JSON line: cell-interactions/s (M2L pairs per second) of the FP64
+pass, the roofline of the dominant kernel (FP64 VALU), a CPU baseline (the
+reference's own interaction templates -- or the oracle when the reference
+tree was not available at build time -- on the host cores, over the same
+level), and the mixed-precision form on the same level and on the
+4096-sub-grid level with its error against the FP64 path.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--config c2|c4] [--precision fp64]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-c4] [--cpu-repeats R]
 """
 import argparse
 import json
@@ -31,12 +34,85 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c2", "c4"])
-    ap.add_argument("--cpu-sample", type=int, default=64, help="target sub-grids timed on the CPU (0 = skip)")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 4096-sub-grid FP64/mixed comparison")
+    ap.add_argument("--cpu-repeats", type=int, default=3, help="passes of the CPU baseline over the level (0 = skip)")
     ap.add_argument("--no-check", action="store_true", help="skip the post-run parity spot check")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run 3 device-to-device copies of this many MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def time_plan(plan, stream, steps, warmup, hip):
+    for _ in range(warmup):
+        plan.execute(stream.handle)
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        plan.execute(stream.handle)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    e0.destroy()
+    e1.destroy()
+    return ms
+
+
+def one_pass(plan, dl, stream):
+    dl.zero_outputs()
+    plan.execute(stream.handle)
+    stream.synchronize()
+    return dl.L.numpy(), dl.L_c.numpy()
+
+
+def mixed_errors(lv, Lm, Lcm, Lr, Lcr):
+    """Error of the mixed-precision pass against the FP64 pass over all
+    target cells: per component normwise (max cell error / max |ref|),
+    elementwise relative, and total force / torque."""
+    import numpy as np
+
+    from metrics import force_torque_error
+    tg = lv.targets
+    a = np.concatenate([Lm[tg], Lcm[tg]], axis=-1).reshape(-1, 23)
+    b = np.concatenate([Lr[tg], Lcr[tg]], axis=-1).reshape(-1, 23)
+    scale = np.abs(b).max(axis=0)
+    normwise = np.abs(a - b).max(axis=0) / np.where(scale > 0, scale, 1.0)
+    elem = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+    ef, et = force_torque_error(Lm[tg], Lcm[tg], Lr[tg], Lcr[tg], lv.M[tg, :, 0], lv.com[tg])
+    return {
+        "max_normwise_rel_err": float(normwise.max()),
+        "normwise_rel_err_per_component": [float(x) for x in normwise],
+        "max_elementwise_rel_err": float(elem.max()),
+        "median_elementwise_rel_err": float(np.median(elem)),
+        "force_rel_err": ef,
+        "torque_rel_err": et,
+    }
+
+
+def run_level(ctx, lv, args, hip, _abi, with_mixed=True):
+    dl = hip.LevelOnDevice(lv)
+    stream = hip.Stream()
+    out = {}
+    p64 = dl.plan(ctx, type_=_abi.RHO, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
+    st64 = p64.stats()
+    ms64 = time_plan(p64, stream, args.steps, args.warmup, hip)
+    stage_ms = p64.time_stages(stream.handle, iters=max(3, min(args.steps, 20)))
+    L64, Lc64 = one_pass(p64, dl, stream)
+    out["fp64"] = {"ms_per_step": ms64, "stage_ms": stage_ms, "stats": st64}
+    if with_mixed:
+        pm = dl.plan(ctx, type_=_abi.RHO, precision=_abi.PRECISION_MIXED, mixed_far_d2=4,
+                     flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
+        stm = pm.stats()
+        msm = time_plan(pm, stream, args.steps, args.warmup, hip)
+        stage_m = pm.time_stages(stream.handle, iters=max(3, min(args.steps, 20)))
+        Lm, Lcm = one_pass(pm, dl, stream)
+        out["mixed"] = {"ms_per_step": msm, "stage_ms": stage_m, "stats": stm,
+                        "errors": mixed_errors(lv, Lm, Lcm, L64, Lc64)}
+        pm.close()
+    p64.close()
+    stream.destroy()
+    dl.free()
+    return out, (L64, Lc64)
 
 
 def main():
@@ -49,14 +125,12 @@ def main():
 
     from octotiger_amd import _abi, hip, native, workloads
 
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+
     if hip.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(3)
-    lv = workloads.config2() if args.config == "c2" else workloads.config4()
     ctx = native.Context(theta=0.5, device=0)
-    dl = hip.LevelOnDevice(lv)
-    plan = dl.plan(ctx, type_=_abi.RHO, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
-    stats = plan.stats()
     if args.copy_calibration_mib > 0:
         nb = args.copy_calibration_mib << 20
         src, dst = hip.DeviceArray((nb // 8,)), hip.DeviceArray((nb // 8,))
@@ -66,29 +140,18 @@ def main():
         hip.synchronize()
         src.free()
         dst.free()
-    stream = hip.Stream()
-    sh = stream.handle
-    for _ in range(args.warmup):
-        plan.execute(sh)
-    stream.synchronize()
-    e0, e1 = hip.Event(), hip.Event()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        plan.execute(sh)
-    e1.record(stream)
-    e1.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = e0.elapsed_ms(e1) / args.steps
-    pairs = stats["far_pairs"] + stats["near_pairs"]
-    value = pairs / (ms_step * 1e-3)
 
-    # dominant kernel: the level interaction kernel, timed alone with HIP
-    # events on the stream it is launched on
-    stage_ms = plan.time_stages(sh, iters=max(3, min(args.steps, 20)))
-    k_ms = stage_ms[1]
-    flops = stats["flops"]
-    achieved = flops / (k_ms * 1e-3) / 1e12
+    lv = workloads.config2()
+    t0 = time.perf_counter()
+    c2, (L64, Lc64) = run_level(ctx, lv, args, hip, _abi)
+    wall = time.perf_counter() - t0
+    st = c2["fp64"]["stats"]
+    pairs = st["far_pairs"] + st["near_pairs"]
+    ms_step = c2["fp64"]["ms_per_step"]
+    value = pairs / (ms_step * 1e-3)
+    k_ms = c2["fp64"]["stage_ms"][1]
+    achieved = st["flops"] / (k_ms * 1e-3) / 1e12
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{lv.name}.json")
     if os.path.exists(tpath):
@@ -98,33 +161,52 @@ def main():
     check = None
     if not args.no_check:
         import oracle
-        # the accumulated L is (warmup + steps + stage timing) times one pass;
-        # one fresh pass on a few targets is compared with the oracle
-        dl.zero_outputs()
-        plan.execute(sh)
-        stream.synchronize()
         sample = lv.targets[:: max(1, len(lv.targets) // 8)][:8]
         Lr, Lcr = oracle.level_interactions(lv, targets=sample)
-        Lg, Lcg = dl.L.numpy(), dl.L_c.numpy()
         err = 0.0
-        for a, b in ((Lg[sample], Lr[sample]), (Lcg[sample], Lcr[sample])):
+        for a, b in ((L64[sample], Lr[sample]), (Lc64[sample], Lcr[sample])):
             a2, b2 = a.reshape(-1, a.shape[-1]), b.reshape(-1, b.shape[-1])
             for c in range(b2.shape[1]):
                 err = max(err, float(np.abs(a2[:, c] - b2[:, c]).max() / np.abs(b2[:, c]).max()))
-        check = {"targets": int(len(sample)), "max_normwise_rel_err": err}
+        check = {"targets": int(len(sample)), "max_normwise_rel_err_vs_oracle": err}
 
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_repeats > 0:
         import oracle
+        use_ref = oracle.ref_available()
         ncores = min(16, os.cpu_count() or 1)
-        sample = lv.targets[: args.cpu_sample]
         t = time.perf_counter()
-        oracle.level_interactions(lv, targets=sample, nthreads=ncores)
-        dt = time.perf_counter() - t
-        cpu = {"value": len(sample) * (pairs / len(lv.targets)) / dt, "unit": "cell-interactions/s",
-               "cores": ncores, "kind": "port",
-               "sample": f"{len(sample)} of the {len(lv.targets)} target sub-grids of the same level, "
-                         f"FP64 scalar oracle (Kokkos host-kernel walk), OpenMP over sub-grids, {dt:.2f} s"}
+        for _ in range(args.cpu_repeats):
+            oracle.level_interactions(lv, nthreads=ncores, use_ref=use_ref)
+        dt = (time.perf_counter() - t) / args.cpu_repeats
+        cpu = {"value": pairs / dt, "unit": "cell-interactions/s", "cores": ncores,
+               "kind": "reference" if use_ref else "port",
+               "sample": (f"the whole level ({len(lv.targets)} sub-grids, {pairs} pairs) x {args.cpu_repeats} passes, "
+                          f"{dt:.3f} s per pass on {ncores} threads = {dt * ncores * args.cpu_repeats:.1f} core-s; "
+                          + ("Octo-Tiger's own FP64 interaction templates (compute_kernel_templates.hpp) compiled "
+                             "with g++ -O2, walked in the Kokkos host-kernel order with SIMD width 1"
+                             if use_ref else "FP64 scalar oracle (restated Kokkos host-kernel walk)")
+                          + ", one sub-grid per OpenMP task")}
+
+    mixed = {}
+    m2 = c2["mixed"]
+    mixed["uniform-level-512"] = {
+        "value": pairs / (m2["ms_per_step"] * 1e-3), "ms_per_step": m2["ms_per_step"],
+        "speedup_vs_fp64": ms_step / m2["ms_per_step"], "kernel_ms": m2["stage_ms"][1],
+        "fp32_far_pairs": m2["stats"]["mixed_far_pairs"], "errors": m2["errors"]}
+    if not args.no_c4:
+        lv4 = workloads.config4()
+        c4, _ = run_level(ctx, lv4, args, hip, _abi)
+        s4 = c4["fp64"]["stats"]
+        p4 = s4["far_pairs"] + s4["near_pairs"]
+        mixed["uniform-level-4096"] = {
+            "fp64_value": p4 / (c4["fp64"]["ms_per_step"] * 1e-3), "fp64_ms_per_step": c4["fp64"]["ms_per_step"],
+            "fp64_kernel_ms": c4["fp64"]["stage_ms"][1],
+            "fp64_valu_frac": s4["flops"] / (c4["fp64"]["stage_ms"][1] * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
+            "value": p4 / (c4["mixed"]["ms_per_step"] * 1e-3), "ms_per_step": c4["mixed"]["ms_per_step"],
+            "kernel_ms": c4["mixed"]["stage_ms"][1],
+            "speedup_vs_fp64": c4["fp64"]["ms_per_step"] / c4["mixed"]["ms_per_step"],
+            "fp32_far_pairs": c4["mixed"]["stats"]["mixed_far_pairs"], "errors": c4["mixed"]["errors"]}
 
     out = {
         "metric": METRIC,
@@ -144,10 +226,10 @@ def main():
                         f"({lv.nslots} slots), theta=0.5, RHO (angular-momentum correction on), one "
                         f"hipGraph per pass",
             "subgrids": int(len(lv.targets)),
-            "far_pairs_per_pass": int(stats["far_pairs"]),
+            "far_pairs_per_pass": int(st["far_pairs"]),
             "flops_per_pair": FLOPS_PER_PAIR["rho"],
             "precision": "fp64",
-            "workgroup_waves": int(stats["variant"]),
+            "kernel_variant": int(st["variant"]),
         },
         "roofline": {
             "bound": "valu",
@@ -158,16 +240,14 @@ def main():
             "frac": achieved / FP64_VALU_PEAK_TFLOPS,
             "traffic": traffic,
             "kernel_ms": k_ms,
-            "staging_ms": stage_ms[0],
+            "staging_ms": c2["fp64"]["stage_ms"][0],
         },
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
         "parity_check": check,
-        "wall_s_timed": wall,
+        "wall_s_c2": wall,
     }
     print(json.dumps(out), flush=True)
-    plan.close()
-    dl.free()
-    stream.destroy()
     ctx.close()
 
 

@@ -210,6 +210,108 @@ __device__ __forceinline__ void m2l_far_rec(const Rec& r, double X0, double X1, 
     }
 }
 
+// Mixed precision far pair: the separation is formed in FP64 and rounded to
+// FP32; the derivative tensor and the 20-component contraction run in FP32 and
+// accumulate into the FP32 partial sums Lf of the current coarse offset (at
+// most 8 pairs, added to the FP64 expansion by the caller).  The
+// angular-momentum correction stays FP64: 1/r is refined from the FP32
+// estimate by one FP64 Newton step and the octupole difference is formed from
+// the FP64 record.
+template <bool RHO>
+__device__ __forceinline__ void m2l_far_mixed(const Rec& r, double X0, double X1, double X2, double inv_mc0,
+    const double* mc, float* Lf, double* C) {
+    const double xd = X0 - r.v[10].x, yd = X1 - r.v[10].y, zd = X2 - r.v[11].x;
+    const float x = (float)xd, y = (float)yd, z = (float)zd;
+    float m[20];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        m[2 * i] = (float)r.v[i].x;
+        m[2 * i + 1] = (float)r.v[i].y;
+    }
+    const float x00 = x * x, x11 = y * y, x22 = z * z;
+    float r2 = fmaf(z, z, fmaf(y, y, x00));
+    r2 = r2 < 1.0e-20f ? 1.0e-20f : r2;
+    const float ri = __builtin_amdgcn_rsqf(r2);
+    const float r2inv = ri * ri;
+    const float d1 = ri * r2inv;
+    const float d2 = (-3.0f * r2inv) * d1;
+    const float d3 = (-5.0f * r2inv) * d2;
+    float D[20];
+    D[0] = -ri;
+    D[1] = x * d1;
+    D[2] = y * d1;
+    D[3] = z * d1;
+    D[4] = fmaf(d2, x00, d1);
+    D[5] = d2 * (x * y);
+    D[6] = d2 * (x * z);
+    D[7] = fmaf(d2, x11, d1);
+    D[8] = d2 * (y * z);
+    D[9] = fmaf(d2, x22, d1);
+    const float d2x = d2 * x, d2y = d2 * y, d2z = d2 * z;
+    const float d3x00 = d3 * x00, d3x11 = d3 * x11, d3x22 = d3 * x22;
+    D[10] = fmaf(d3x00, x, 3.0f * d2x);
+    D[11] = fmaf(d3x00, y, d2y);
+    D[12] = fmaf(d3x00, z, d2z);
+    D[13] = fmaf(d3x11, x, d2x);
+    D[14] = (d3 * x) * (y * z);
+    D[15] = fmaf(d3x22, x, d2x);
+    D[16] = fmaf(d3x11, y, 3.0f * d2y);
+    D[17] = fmaf(d3x11, z, d2z);
+    D[18] = fmaf(d3x22, y, d2y);
+    D[19] = fmaf(d3x22, z, 3.0f * d2z);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Lf[i] = fmaf(m[0], D[i], Lf[i]);
+    if constexpr (!RHO) {
+        Lf[0] = fmaf(-m[1], D[1], Lf[0]);
+        Lf[0] = fmaf(-m[2], D[2], Lf[0]);
+        Lf[0] = fmaf(-m[3], D[3], Lf[0]);
+        Lf[1] = fmaf(-m[1], D[4] + D[5] + D[6], Lf[1]);
+        Lf[2] = fmaf(-m[2], D[5] + D[7] + D[8], Lf[2]);
+        Lf[3] = fmaf(-m[3], D[6] + D[8] + D[9], Lf[3]);
+    }
+    {
+        float l0 = Lf[0], l1 = Lf[1], l2 = Lf[2], l3 = Lf[3];
+        l0 = fmaf(m[4], D[4], l0);   l1 = fmaf(m[4], D[10], l1);  l2 = fmaf(m[4], D[11], l2);  l3 = fmaf(m[4], D[12], l3);
+        l0 = fmaf(m[5], D[5], l0);   l1 = fmaf(m[5], D[11], l1);  l2 = fmaf(m[5], D[13], l2);  l3 = fmaf(m[5], D[14], l3);
+        l0 = fmaf(m[6], D[6], l0);   l1 = fmaf(m[6], D[12], l1);  l2 = fmaf(m[6], D[14], l2);  l3 = fmaf(m[6], D[15], l3);
+        l0 = fmaf(m[7], D[7], l0);   l1 = fmaf(m[7], D[13], l1);  l2 = fmaf(m[7], D[16], l2);  l3 = fmaf(m[7], D[17], l3);
+        l0 = fmaf(m[8], D[8], l0);   l1 = fmaf(m[8], D[14], l1);  l2 = fmaf(m[8], D[17], l2);  l3 = fmaf(m[8], D[18], l3);
+        l0 = fmaf(m[9], D[9], l0);   l1 = fmaf(m[9], D[15], l1);  l2 = fmaf(m[9], D[18], l2);  l3 = fmaf(m[9], D[19], l3);
+#pragma unroll
+        for (int k = 10; k < 20; ++k) l0 = fmaf(-m[k], D[k], l0);
+        Lf[0] = l0; Lf[1] = l1; Lf[2] = l2; Lf[3] = l3;
+        static constexpr int kRowB[6][3] = {{10, 11, 12}, {11, 13, 14}, {12, 14, 15}, {13, 16, 17}, {14, 17, 18},
+            {15, 18, 19}};
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            float t = fmaf(m[0], D[4 + i], Lf[4 + i]);
+            t = fmaf(-m[1], D[kRowB[i][0]], t);
+            t = fmaf(-m[2], D[kRowB[i][1]], t);
+            Lf[4 + i] = fmaf(-m[3], D[kRowB[i][2]], t);
+        }
+#pragma unroll
+        for (int i = 10; i < 20; ++i) Lf[i] = fmaf(m[0], D[i], Lf[i]);
+    }
+    if constexpr (RHO) {
+        double r2d = fma(zd, zd, fma(yd, yd, xd * xd));
+        r2d = r2d < 1.0e-20 ? 1.0e-20 : r2d;
+        double rid = (double)ri;
+        rid = fma(rid, fma(-(0.5 * r2d) * rid, rid, 0.5), rid);
+        const double r2i = rid * rid;
+        const double e1 = rid * r2i;
+        const double e2 = (-3.0 * r2i) * e1;
+        const double e3 = (-5.0 * r2i) * e2;
+        const double n0c = r.v[0].x * inv_mc0;
+        double a[10];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+            const double mk = (k & 1) ? r.v[5 + (k >> 1)].y : r.v[5 + (k >> 1)].x;
+            a[k] = fma(-mc[k], n0c, mk);
+        }
+        am_correction(xd, yd, zd, e2, e3, a, C);
+    }
+}
+
 template <bool RHO>
 __device__ __forceinline__ void m2l_far(const double* __restrict__ p, double X0, double X1, double X2,
     double inv_mc0, const double* mc, double* L, double* C) {
@@ -290,6 +392,7 @@ struct LevelArgs {
     const int32_t* kind;    // [nslots]
     const int2* items;      // (slot, flags)
     const Tables* tab;
+    const uint64_t* mixbits;   // per coarse offset: byte k, bit j = far pair evaluated in FP32
     int32_t nitems;
     int32_t pad;
     double inv_dx;

@@ -125,7 +125,7 @@ __device__ __forceinline__ int partner_base(const LaneGeom& g, int ax, int ay, i
     return ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
 }
 
-template <bool RHO>
+template <bool RHO, bool MIXED>
 __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind, int w) {
     const LaneGeom g = lane_geom(w);
     const Tables* __restrict__ tab = a.tab;
@@ -153,20 +153,48 @@ __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_n
     unsigned fb;
     const double* pb = far_base(0, fb);
     Rec nxt;
-    load_rec(pb, nxt);
+    if constexpr (!MIXED) load_rec(pb, nxt);
+    float Lf[20];
+    if constexpr (MIXED) {
+#pragma unroll
+        for (int i = 0; i < 20; ++i) Lf[i] = 0.0f;
+    }
     for (int di = 0; di < nd; ++di) {
         unsigned fbn = 0;
         const double* pbn = pb;
+        const unsigned mb = MIXED ? (unsigned)(a.mixbits[di] >> (8 * g.k)) & 0xffu : 0u;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const Rec cur = nxt;
-            if (j < 7) {
-                load_rec(pb + (j + 1) * kRecDoubles, nxt);
-            } else if (di + 1 < nd) {
-                pbn = far_base(di + 1, fbn);
-                load_rec(pbn, nxt);
+            // FP64: the next record is prefetched while this pair computes;
+            // MIXED keeps a single record live (its FP32 partial sums need
+            // the registers)
+            Rec cur;
+            if constexpr (MIXED) {
+                load_rec(pb + j * kRecDoubles, cur);
+                if (j == 7 && di + 1 < nd) pbn = far_base(di + 1, fbn);
+            } else {
+                cur = nxt;
+                if (j < 7) {
+                    load_rec(pb + (j + 1) * kRecDoubles, nxt);
+                } else if (di + 1 < nd) {
+                    pbn = far_base(di + 1, fbn);
+                    load_rec(pbn, nxt);
+                }
             }
-            if (fb & (1u << j)) m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
+            if (fb & (1u << j)) {
+                if (MIXED && (mb & (1u << j)))
+                    m2l_far_mixed<RHO>(cur, X0, X1, X2, inv_mc0, mc, Lf, C);
+                else
+                    m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
+            }
+        }
+        if constexpr (MIXED) {
+            // the coarse offset's FP32 partial sums join the FP64 expansion
+#pragma unroll
+            for (int i = 0; i < 20; ++i) {
+                L[i] += (double)Lf[i];
+                Lf[i] = 0.0f;
+            }
         }
         pb = pbn;
         fb = fbn;
@@ -449,7 +477,7 @@ __device__ void leaf_item(const LevelArgs& a, int slot, int flags, const int* s_
 // One target sub-grid is split over 8 / WAVES workgroups of WAVES waves.  The
 // workgroup index is first spread over the XCDs (contiguous runs per XCD),
 // so the parts of one sub-grid and its neighbours share an L2.
-template <bool RHO, int WAVES, int MAPPING>
+template <bool RHO, int WAVES, int MAPPING, bool MIXED = false>
 __global__ __launch_bounds__(64 * WAVES) void level_kernel(LevelArgs a) {
     constexpr int kParts = 8 / WAVES;
     __shared__ int s_nbr[27];
@@ -472,14 +500,22 @@ __global__ __launch_bounds__(64 * WAVES) void level_kernel(LevelArgs a) {
     else if (MAPPING == 2)
         m2l_item_lds<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage + (threadIdx.x >> 6) * kStageWave);
     else
-        m2l_item<RHO>(a, it.x, it.y, s_nbr, s_kind, w);
+        m2l_item<RHO, MIXED>(a, it.x, it.y, s_nbr, s_kind, w);
 }
 
 template <bool RHO>
-void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la) {
+void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, bool mixed = false) {
     const int waves = variant % 100;
     const int parts = 8 / waves;
     const dim3 grid(nitems * parts), block(64 * waves);
+    if (mixed) {
+        switch (variant) {
+        case 8: hipLaunchKernelGGL((level_kernel<RHO, 8, 0, true>), grid, block, 0, st, la); break;
+        case 4: hipLaunchKernelGGL((level_kernel<RHO, 4, 0, true>), grid, block, 0, st, la); break;
+        default: hipLaunchKernelGGL((level_kernel<RHO, 2, 0, true>), grid, block, 0, st, la); break;
+        }
+        return;
+    }
     switch (variant) {
     case 8: hipLaunchKernelGGL((level_kernel<RHO, 8, 0>), grid, block, 0, st, la); break;
     case 4: hipLaunchKernelGGL((level_kernel<RHO, 4, 0>), grid, block, 0, st, la); break;
@@ -598,6 +634,7 @@ struct octofmm_plan {
     int2* d_items = nullptr;
     octofmm_plan_stats stats;
     int variant = 8;           // level-kernel variant (see kVariants)
+    uint64_t* d_mixbits = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 };
@@ -816,6 +853,7 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         la.kind = c->d_meta + kMetaKind;
         la.items = reinterpret_cast<const int2*>(c->d_meta + kMetaItems);
         la.tab = c->d_tab;
+        la.mixbits = nullptr;
         la.nitems = 1;
         la.inv_dx = 1.0 / dx;
         la.neg_inv_dx = -1.0 / dx;
@@ -1005,13 +1043,14 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         la.kind = p->d_kind;
         la.items = p->d_items;
         la.tab = c->d_tab;
+        la.mixbits = p->d_mixbits;
         la.nitems = p->nitems;
         la.inv_dx = 1.0 / p->dx;
         la.neg_inv_dx = -1.0 / p->dx;
         if (p->type == OCTOFMM_RHO)
-            launch_level<true>(p->variant, p->nitems, st, la);
+            launch_level<true>(p->variant, p->nitems, st, la, p->precision == OCTOFMM_PRECISION_MIXED);
         else
-            launch_level<false>(p->variant, p->nitems, st, la);
+            launch_level<false>(p->variant, p->nitems, st, la, p->precision == OCTOFMM_PRECISION_MIXED);
         HIPC(c, hipGetLastError());
     }
     return OCTOFMM_OK;
@@ -1055,6 +1094,7 @@ static octofmm_status plan_autotune_run(octofmm_plan* p) {
     float best = 1e30f;
     int best_w = 8;
     for (int w : kVariants) {
+        if (p->precision == OCTOFMM_PRECISION_MIXED && w >= 100) continue;
         p->variant = w;
         s = plan_launch(p, st, 2);   // warm-up
         if (s != OCTOFMM_OK) return s;
@@ -1197,11 +1237,47 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
         (!items.empty() &&
             hipMemcpy(p->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess))
         return cleanup(OCTOFMM_ERR_HIP, "plan table upload failed");
+    if (p->precision == OCTOFMM_PRECISION_MIXED) {
+        // far pairs with |d|^2 > mixed_far_d2 (cell units) are evaluated in FP32
+        std::vector<uint64_t> mb(T.ndelta > 0 ? T.ndelta : 1, 0);
+        for (int di = 0; di < T.ndelta; ++di)
+            for (int k = 0; k < 8; ++k)
+                for (int j = 0; j < 8; ++j) {
+                    const int bit = 8 * k + j;
+                    if (!((T.farbits[di] >> bit) & 1)) continue;
+                    const int dx = 2 * T.delta[di][0] + ((j >> 2) & 1) - ((k >> 2) & 1);
+                    const int dy = 2 * T.delta[di][1] + ((j >> 1) & 1) - ((k >> 1) & 1);
+                    const int dz = 2 * T.delta[di][2] + (j & 1) - (k & 1);
+                    if (dx * dx + dy * dy + dz * dz > lv->mixed_far_d2) mb[di] |= uint64_t(1) << bit;
+                }
+        if (hipMalloc((void**)&p->d_mixbits, sizeof(uint64_t) * mb.size()) != hipSuccess ||
+            hipMemcpy(p->d_mixbits, mb.data(), sizeof(uint64_t) * mb.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(OCTOFMM_ERR_ALLOCATION, "mixed-precision table upload failed");
+        // FP32 pair count (per target cell class, summed over the M2L items)
+        int64_t per_cell[8] = {0};
+        for (int k = 0; k < 8; ++k)
+            for (int di = 0; di < T.ndelta; ++di) per_cell[k] += __builtin_popcountll((mb[di] >> (8 * k)) & 0xffull);
+        p->stats.mixed_far_pairs = 0;
+        for (const int2& it : items) {
+            if (it.y & kItemLeaf) continue;
+            for (int cx = 0; cx < 4; ++cx)
+                for (int cy = 0; cy < 4; ++cy)
+                    for (int cz = 0; cz < 4; ++cz)
+                        for (int di = 0; di < T.ndelta; ++di) {
+                            const int px = cx + T.delta[di][0], py = cy + T.delta[di][1], pz = cz + T.delta[di][2];
+                            const int raw = (px < 0 ? 0 : px < 4 ? 1 : 2) + 3 * (py < 0 ? 0 : py < 4 ? 1 : 2) +
+                                9 * (pz < 0 ? 0 : pz < 4 ? 1 : 2);
+                            if (lv->neighbors[(size_t)it.x * 27 + raw] < 0) continue;
+                            p->stats.mixed_far_pairs += __builtin_popcountll(mb[di]);
+                        }
+        }
+        (void)per_cell;
+    }
     const char* forced = std::getenv("OCTOFMM_VARIANT");
     bool forced_ok = false;
     if (forced)
         for (int v : kVariants)
-            if (std::atoi(forced) == v) {
+            if (std::atoi(forced) == v && !(p->precision == OCTOFMM_PRECISION_MIXED && v >= 100)) {
                 p->variant = v;
                 forced_ok = true;
             }

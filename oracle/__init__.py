@@ -62,6 +62,9 @@ def ref_lib():
         R.ref_multipole_interactions.argtypes = [ctypes.c_double, ctypes.POINTER(_abi.Subgrid), _dp, _dp]
         R.ref_monopole_interactions.argtypes = [ctypes.c_double, ctypes.POINTER(_abi.Subgrid),
                                                 ctypes.c_int32, _dp, _dp]
+        R.ref_level_interactions.argtypes = [
+            ctypes.c_double, ctypes.c_int32, _ip, _ip, _dp, ctypes.c_double, ctypes.c_int32, _ip,
+            ctypes.c_int32, _dp, _dp, _dp, _dp, _dp, ctypes.c_int32]
         _ref = R
     return _ref
 
@@ -136,14 +139,16 @@ def monopole_interactions(subgrid, contains_multipole_neighbor, theta=0.5, L=Non
     return L, L_c
 
 
-def level_interactions(level, theta=0.5, type_=_abi.RHO, targets=None, nthreads=None):
-    """Oracle over every target slot of a workloads.Level; returns (L, L_c)
-    arrays shaped like level.M / level.com (zeros outside targets)."""
+def level_interactions(level, theta=0.5, type_=_abi.RHO, targets=None, nthreads=None, use_ref=False):
+    """Oracle (or, with use_ref, the reference-template harness) over every
+    target slot of a workloads.Level; returns (L, L_c) arrays shaped like
+    level.M / level.com (zeros outside targets)."""
     targets = level.targets if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
     L = np.zeros((level.nslots, 512, 20))
     L_c = np.zeros((level.nslots, 512, 3))
     nthreads = nthreads or min(8, os.cpu_count() or 1)
-    st = lib().oracle_level_interactions(
+    f = ref_lib().ref_level_interactions if use_ref else lib().oracle_level_interactions
+    st = f(
         theta, level.nslots, _abi.iptr(level.neighbors), _abi.iptr(level.kind), _abi.dptr(level.xbase),
         level.dx, int(targets.size), _abi.iptr(targets), type_, _abi.dptr(level.M),
         _abi.dptr(level.com), _abi.dptr(level.mon), _abi.dptr(L), _abi.dptr(L_c), nthreads)

@@ -463,3 +463,55 @@ int ref_monopole_interactions(double theta, const octofmm_subgrid* in, int32_t w
 }
 
 }   // extern "C"
+
+// Level loop over the reference templates: every target slot of a level, each
+// through ref_multipole_interactions / ref_monopole_interactions, OpenMP over
+// targets (one sub-grid per task, as the reference schedules one task per
+// octree node).
+extern "C" int ref_level_interactions(double theta, int32_t nslots, const int32_t* neighbors,
+    const int32_t* kind, const double* xbase, double dx, int32_t ntargets, const int32_t* targets,
+    int32_t type, const double* M, const double* com, const double* mon, double* L, double* L_c,
+    int32_t nthreads) {
+    int status = OCTOFMM_OK;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int t = 0; t < ntargets; ++t) {
+        const int s = targets[t];
+        if (s < 0 || s >= nslots) continue;
+        octofmm_subgrid in;
+        std::memset(&in, 0, sizeof(in));
+        in.M = M + (size_t)s * 512 * 20;
+        in.com = com + (size_t)s * 512 * 3;
+        in.mon = mon + (size_t)s * 512;
+        in.dx = dx;
+        for (int k = 0; k < 3; ++k) in.xbase[k] = xbase[3 * s + k];
+        in.type = type;
+        bool has_multipole = false;
+        for (int f = 0; f < 26; ++f) {
+            const int raw = f < 13 ? f : f + 1;
+            const int n = neighbors[(size_t)s * 27 + raw];
+            octofmm_neighbor& nb = in.neighbors[f];
+            if (n < 0) {
+                nb.kind = OCTOFMM_NEIGHBOR_EMPTY;
+            } else if (kind[n] == OCTOFMM_SLOT_LEAF) {
+                nb.kind = OCTOFMM_NEIGHBOR_MONOPOLE;
+                nb.count = 512;
+                nb.mon = mon + (size_t)n * 512;
+            } else {
+                nb.kind = OCTOFMM_NEIGHBOR_MULTIPOLE;
+                nb.count = 512;
+                nb.M = M + (size_t)n * 512 * 20;
+                nb.com = com + (size_t)n * 512 * 3;
+                has_multipole = true;
+            }
+        }
+        double* Ls = L + (size_t)s * 512 * 20;
+        double* Lcs = L_c + (size_t)s * 512 * 3;
+        const int st = kind[s] == OCTOFMM_SLOT_LEAF ? ref_monopole_interactions(theta, &in, has_multipole, Ls, Lcs)
+                                                    : ref_multipole_interactions(theta, &in, Ls, Lcs);
+        if (st != OCTOFMM_OK) {
+#pragma omp critical
+            status = st;
+        }
+    }
+    return status;
+}
