@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-c4", action="store_true", help="skip the 4096-sub-grid FP64/mixed comparison")
+    ap.add_argument("--no-c3", action="store_true", help="skip the mixed-kind 2048-sub-grid level")
     ap.add_argument("--cpu-repeats", type=int, default=3, help="passes of the CPU baseline over the level (0 = skip)")
     ap.add_argument("--no-check", action="store_true", help="skip the post-run parity spot check")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -208,6 +209,20 @@ def main():
             "speedup_vs_fp64": c4["fp64"]["ms_per_step"] / c4["mixed"]["ms_per_step"],
             "fp32_far_pairs": c4["mixed"]["stats"]["mixed_far_pairs"], "errors": c4["mixed"]["errors"]}
 
+    kinds = None
+    if not args.no_c3:
+        lv3 = workloads.config3()
+        c3, _ = run_level(ctx, lv3, args, hip, _abi, with_mixed=False)
+        s3 = c3["fp64"]["stats"]
+        ms3 = c3["fp64"]["ms_per_step"]
+        p3 = s3["far_pairs"] + s3["near_pairs"] + s3["p2p_pairs"] + s3["p2m_pairs"]
+        kinds = {"workload": f"{lv3.name}: {len(lv3.targets)} sub-grids, {lv3.info}",
+                 "ms_per_step": ms3, "kernel_ms": c3["fp64"]["stage_ms"][1],
+                 "cell_interactions_per_s": p3 / (ms3 * 1e-3),
+                 "pairs": {k: int(s3[k]) for k in ("far_pairs", "near_pairs", "p2p_pairs", "p2m_pairs")},
+                 "valu_frac": s3["flops"] / (c3["fp64"]["stage_ms"][1] * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
+                 "kernel_variant": int(s3["variant"])}
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -244,6 +259,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "mixed_kind_level": kinds,
         "parity_check": check,
         "wall_s_c2": wall,
     }

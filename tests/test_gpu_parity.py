@@ -210,3 +210,20 @@ def test_mixed_precision_far_field(ctx, t):
         ef, et = force_torque_error(L[tg], Lc[tg], Lr[tg], Lcr[tg], lv.M[tg, :, 0], lv.com[tg])
         print(f"mixed force {ef:.3e} torque {et:.3e} Lc {normwise(Lc[tg], Lcr[tg]):.3e}", flush=True)
         assert ef <= MIXED_FORCE_TOL and et <= MIXED_FORCE_TOL
+
+
+def test_level_config3_full_size(ctx):
+    """2048 sub-grids of mixed kind (leaf p2p/p2m, non-leaf interior, non-leaf
+    next to leaves) in one pass against the oracle."""
+    lv = workloads.config3()
+    print(f"config3 {lv.info}", flush=True)
+    outs, stats = run_level(ctx, lv, _abi.RHO)
+    L, Lc = outs[0]
+    assert stats["n_m2l_items"] + stats["n_leaf_items"] == 2048
+    assert stats["near_pairs"] > 0 and stats["p2p_pairs"] > 0 and stats["p2m_pairs"] > 0
+    Lr, Lcr = oracle.level_interactions(lv)
+    tg = lv.targets
+    assert normwise(L[tg], Lr[tg]) <= TOL and normwise(Lc[tg], Lcr[tg]) <= TOL
+    mass = np.where(lv.kind[tg, None] == _abi.SLOT_LEAF, lv.mon[tg], lv.M[tg, :, 0])
+    ef, et = force_torque_error(L[tg], Lc[tg], Lr[tg], Lcr[tg], mass, lv.com[tg])
+    assert ef <= TOL and et <= TOL
