@@ -13,7 +13,10 @@ tree was not available at build time -- on the host cores, over the same
 level), and the mixed-precision form on the same level and on the
 4096-sub-grid level with its error against the FP64 path.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-c4] [--cpu-repeats R]
+It also reports the mixed-kind 2048-sub-grid level and the full adaptive-tree
+solve of a polytrope (10,697 sub-grids, M2M + interactions + L2L + forces).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-c3] [--no-c4] [--no-c5] [--cpu-repeats R]
 """
 import argparse
 import json
@@ -36,6 +39,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-c4", action="store_true", help="skip the 4096-sub-grid FP64/mixed comparison")
     ap.add_argument("--no-c3", action="store_true", help="skip the mixed-kind 2048-sub-grid level")
+    ap.add_argument("--no-c5", action="store_true", help="skip the full adaptive-tree solve")
     ap.add_argument("--cpu-repeats", type=int, default=3, help="passes of the CPU baseline over the level (0 = skip)")
     ap.add_argument("--no-check", action="store_true", help="skip the post-run parity spot check")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -114,6 +118,51 @@ def run_level(ctx, lv, args, hip, _abi, with_mixed=True):
     stream.destroy()
     dl.free()
     return out, (L64, Lc64)
+
+
+def run_tree(ctx, args, hip, _abi, np):
+    """Full adaptive-tree solve (upward M2M, interactions of every level, root
+    stencil on level 0, downward L2L and leaf force assembly) as one hipGraph,
+    FP64 and mixed precision."""
+    from octotiger_amd import workloads
+
+    t = workloads.config5()
+    dev = hip.TreeOnDevice(t)
+    stream = hip.Stream()
+    res = {}
+    for name, prec in (("fp64", _abi.PRECISION_FP64), ("mixed", _abi.PRECISION_MIXED)):
+        plan = dev.plan(ctx, precision=prec, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
+        steps = max(5, args.steps // 5)
+        ms = time_plan(plan, stream, steps, max(1, args.warmup), hip)
+        phases = plan.time_phases(stream.handle, iters=3)
+        plan.execute(stream.handle)
+        stream.synchronize()
+        g = np.concatenate([d["g"].numpy()[lv.kind == _abi.SLOT_LEAF].reshape(-1, 4)
+                            for lv, d in zip(t.levels, dev.levels)])
+        res[name] = {"ms_per_solve": ms, "phases_ms": {"upward_m2m": phases[0], "interactions": phases[1],
+                                                       "downward_l2l_force": phases[2]},
+                     "stats": plan.stats(), "g": g}
+        plan.close()
+    dev.free()
+    st = res["fp64"]["stats"]
+    pairs = st["far_pairs"] + st["near_pairs"] + st["p2p_pairs"] + st["p2m_pairs"] + st["root_pairs"]
+    ga, gb = res["fp64"]["g"], res["mixed"]["g"]
+    ef = float(np.sqrt(((gb[:, 1:] - ga[:, 1:]) ** 2).sum() / (ga[:, 1:] ** 2).sum()))
+    ephi = float(np.abs(gb[:, 0] - ga[:, 0]).max() / np.abs(ga[:, 0]).max())
+    inter = res["fp64"]["phases_ms"]["interactions"]
+    return {
+        "workload": f"{t.name}: n=1.5 polytrope (radius 0.3, 1% log-normal noise), {t.nnodes} sub-grids, "
+                    f"nodes per level {t.info['nodes_per_level']}, leaves per level {t.info['leaves_per_level']}",
+        "fp64": {"ms_per_solve": res["fp64"]["ms_per_solve"], "phases_ms": res["fp64"]["phases_ms"],
+                 "cell_interactions_per_s": pairs / (res["fp64"]["ms_per_solve"] * 1e-3),
+                 "interaction_valu_frac": st["flops"] / (inter * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS},
+        "mixed": {"ms_per_solve": res["mixed"]["ms_per_solve"], "phases_ms": res["mixed"]["phases_ms"],
+                  "speedup_vs_fp64": res["fp64"]["ms_per_solve"] / res["mixed"]["ms_per_solve"],
+                  "fp32_far_pairs": int(res["mixed"]["stats"]["mixed_far_pairs"]),
+                  "leaf_force_rms_rel_err_vs_fp64": ef, "leaf_phi_max_rel_err_vs_fp64": ephi},
+        "pairs": {k: int(st[k]) for k in ("far_pairs", "near_pairs", "p2p_pairs", "p2m_pairs", "root_pairs")},
+        "nodes": int(st["nodes"]), "leaves": int(st["leaves"]),
+    }
 
 
 def main():
@@ -223,6 +272,10 @@ def main():
                  "valu_frac": s3["flops"] / (c3["fp64"]["stage_ms"][1] * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
                  "kernel_variant": int(s3["variant"])}
 
+    tree = None
+    if not args.no_c5:
+        tree = run_tree(ctx, args, hip, _abi, np)
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -260,6 +313,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
         "mixed_kind_level": kinds,
+        "tree_solve": tree,
         "parity_check": check,
         "wall_s_c2": wall,
     }

@@ -181,6 +181,11 @@ typedef struct octofmm_level {
 
 #define OCTOFMM_PLAN_USE_GRAPH 1   /* capture the level pass in one hipGraph */
 #define OCTOFMM_PLAN_AUTOTUNE 2    /* time the kernel variants once and keep the fastest */
+/* The level is the root sub-grid alone (nslots = ntargets = 1, non-leaf):
+ * every pair of its cells except self and near offsets, the root stencil of
+ * octotiger/multipole_interactions/kernel/kokkos_kernel.hpp:324-471, 637-764.
+ * Always FP64. */
+#define OCTOFMM_PLAN_ROOT 4
 
 typedef struct octofmm_plan octofmm_plan;
 
@@ -208,6 +213,81 @@ octofmm_status octofmm_level_plan_stats(const octofmm_plan* plan, octofmm_plan_s
 octofmm_status octofmm_level_plan_time_stages(octofmm_plan* plan, void* stream, int32_t iters,
     double* ms);
 octofmm_status octofmm_level_plan_destroy(octofmm_plan* plan);
+
+/* ---------------------------------------------------------------------------
+ * Whole-tree gravity solve: the passes node_server::compute_fmm runs per node
+ * (src/node_server.cpp:589-798), batched per octree level and captured in
+ * one hipGraph:
+ *   1. upward M2M, finest level first (grid::compute_multipoles,
+ *      src/grid_fmm.cpp:1639-1803, taylor::operator>>=, octotiger/taylor.hpp:222-255);
+ *   2. interactions of every level (level plans; the root sub-grid with the
+ *      root stencil);
+ *   3. downward L2L, root first, with the leaf force assembly
+ *      (grid::compute_expansions, src/grid_fmm.cpp:1524-1637,
+ *      taylor::operator<<=, octotiger/taylor.hpp:263-311).
+ * Level 0 holds the root alone.  Every non-leaf node of level l has exactly 8
+ * children on level l+1; the neighbours of a non-leaf node exist on its level
+ * unless outside the domain (proper nesting).
+ * ------------------------------------------------------------------------- */
+typedef struct octofmm_tree_level {
+    int32_t nnodes;
+    const int32_t* neighbors;  /* host [nnodes][27], as octofmm_level.neighbors */
+    const int32_t* kind;       /* host [nnodes] octofmm_slot_kind */
+    const int32_t* parent;     /* host [nnodes] index of the parent on the level above; -1 on level 0 */
+    const int32_t* octant;     /* host [nnodes] 0..7, which 4^3 cell block of the parent:
+                                  bit 0 x, bit 1 y, bit 2 z (geo::octant, octotiger/geometry.hpp:333-335) */
+    const double* xbase;       /* host [nnodes][3] centre of each node's cell (0,0,0) */
+    double dx;                 /* cell width on this level */
+    /* device arrays, AoS per node, caller-owned */
+    double* d_M;               /* [nnodes][512][20] written by the upward pass (non-leaf nodes) */
+    double* d_com;             /* [nnodes][512][3]  written by the upward pass for RHO (leaf cells:
+                                  geometric centres); DRHODT reuses the RHO pass's values */
+    const double* d_mon;       /* [nnodes][512]     leaf cell masses (input; unused in non-leaf nodes) */
+    double* d_L;               /* [nnodes][512][20] output: expansion of every cell */
+    double* d_L_c;             /* [nnodes][512][3]  output (RHO): angular-momentum correction */
+    double* d_g;               /* [nnodes][512][4]  output, may be NULL: leaf cells' phi = G L0 and
+                                  g = -G (L_1..3 + L_c) (L_c only with correct_am); DRHODT: phi only */
+} octofmm_tree_level;
+
+typedef struct octofmm_tree {
+    int32_t nlevels;
+    const octofmm_tree_level* levels;
+    int32_t type;              /* octofmm_solve_type */
+    int32_t precision;         /* octofmm_precision of the non-root levels */
+    int32_t mixed_far_d2;
+    int32_t flags;             /* OCTOFMM_PLAN_USE_GRAPH, OCTOFMM_PLAN_AUTOTUNE */
+    double G;                  /* gravitational constant of the force assembly */
+    int32_t correct_am;        /* add L_c to g (opts().correct_am_grav) */
+    int32_t reserved;
+} octofmm_tree;
+
+typedef struct octofmm_tree_plan octofmm_tree_plan;
+
+typedef struct octofmm_tree_stats {
+    int64_t far_pairs;         /* M2L far pairs over all non-root levels */
+    int64_t near_pairs;
+    int64_t p2p_pairs;
+    int64_t p2m_pairs;
+    int64_t root_pairs;        /* M2L pairs of the root sub-grid */
+    int64_t flops;             /* nominal FLOPs of the interactions of one solve */
+    int64_t mixed_far_pairs;
+    int32_t nodes;
+    int32_t leaves;
+    int32_t nlevels;
+    int32_t reserved;
+} octofmm_tree_stats;
+
+octofmm_status octofmm_tree_plan_create(octofmm_context* ctx, const octofmm_tree* tree,
+    octofmm_tree_plan** plan);
+/* One full solve on a HIP stream (NULL = default stream); asynchronous.
+ * L and L_c are cleared by the solve itself. */
+octofmm_status octofmm_tree_plan_execute(octofmm_tree_plan* plan, void* stream);
+octofmm_status octofmm_tree_plan_stats(const octofmm_tree_plan* plan, octofmm_tree_stats* stats);
+/* Runs each phase `iters` times between HIP events on `stream`; ms[0] upward
+ * pass, ms[1] interactions (all levels), ms[2] downward pass and assembly. */
+octofmm_status octofmm_tree_plan_time_phases(octofmm_tree_plan* plan, void* stream, int32_t iters,
+    double* ms);
+octofmm_status octofmm_tree_plan_destroy(octofmm_tree_plan* plan);
 
 #ifdef __cplusplus
 }

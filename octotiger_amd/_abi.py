@@ -34,6 +34,7 @@ SLOT_LEAF = 1
 
 PLAN_USE_GRAPH = 1
 PLAN_AUTOTUNE = 2
+PLAN_ROOT = 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -103,6 +104,54 @@ class PlanStats(ctypes.Structure):
         ("n_m2l_items", ctypes.c_int32),
         ("n_leaf_items", ctypes.c_int32),
         ("variant", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class TreeLevel(ctypes.Structure):
+    _fields_ = [
+        ("nnodes", ctypes.c_int32),
+        ("neighbors", _ip),
+        ("kind", _ip),
+        ("parent", _ip),
+        ("octant", _ip),
+        ("xbase", _dp),
+        ("dx", ctypes.c_double),
+        ("d_M", ctypes.c_void_p),
+        ("d_com", ctypes.c_void_p),
+        ("d_mon", ctypes.c_void_p),
+        ("d_L", ctypes.c_void_p),
+        ("d_L_c", ctypes.c_void_p),
+        ("d_g", ctypes.c_void_p),
+    ]
+
+
+class Tree(ctypes.Structure):
+    _fields_ = [
+        ("nlevels", ctypes.c_int32),
+        ("levels", ctypes.POINTER(TreeLevel)),
+        ("type", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("mixed_far_d2", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("G", ctypes.c_double),
+        ("correct_am", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class TreeStats(ctypes.Structure):
+    _fields_ = [
+        ("far_pairs", ctypes.c_int64),
+        ("near_pairs", ctypes.c_int64),
+        ("p2p_pairs", ctypes.c_int64),
+        ("p2m_pairs", ctypes.c_int64),
+        ("root_pairs", ctypes.c_int64),
+        ("flops", ctypes.c_int64),
+        ("mixed_far_pairs", ctypes.c_int64),
+        ("nodes", ctypes.c_int32),
+        ("leaves", ctypes.c_int32),
+        ("nlevels", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
     ]
 

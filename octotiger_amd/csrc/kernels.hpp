@@ -415,4 +415,117 @@ struct RootReduceArgs {
     int32_t rho;
 };
 
+// ---------------------------------------------------------------------------
+// Tree passes: expansion shifts of taylor<4> (octotiger/taylor.hpp:222-311)
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int tmap2(int a, int b) {
+    const int i = a < b ? a : b, j = a < b ? b : a;
+    return 4 + i * 3 - (i * (i - 1)) / 2 + (j - i);
+}
+
+__host__ __device__ constexpr int tmap3(int a, int b, int c) {
+    const int t[27] = {10, 11, 12, 11, 13, 14, 12, 14, 15, 11, 13, 14, 13, 16, 17, 14, 17, 18,
+        12, 14, 15, 14, 17, 18, 15, 18, 19};
+    return t[a * 9 + b * 3 + c];
+}
+
+// taylor::operator>>= : B = A moved from its centre by X (X = old - new centre).
+__device__ __forceinline__ void multipole_shift(const double* A, const double* X, double* B) {
+#pragma unroll
+    for (int i = 0; i < 20; ++i) B[i] = A[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) B[1 + a] += A[0] * X[a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = a; b < 3; ++b) {
+            B[tmap2(a, b)] += A[1 + a] * X[b] + X[a] * A[1 + b];
+            B[tmap2(a, b)] += A[0] * X[a] * X[b];
+        }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = a; b < 3; ++b)
+#pragma unroll
+            for (int c = b; c < 3; ++c) {
+                double t = B[tmap3(a, b, c)];
+                t += A[tmap2(a, b)] * X[c] + A[tmap2(b, c)] * X[a] + A[tmap2(c, a)] * X[b];
+                t += A[1 + a] * X[b] * X[c] + A[1 + b] * X[c] * X[a] + A[1 + c] * X[a] * X[b];
+                t += A[0] * X[a] * X[b] * X[c];
+                B[tmap3(a, b, c)] = t;
+            }
+}
+
+// taylor::operator<<= : B = local expansion A re-expanded at +X.
+__device__ __forceinline__ void local_shift(const double* A, const double* X, double* B) {
+#pragma unroll
+    for (int i = 0; i < 20; ++i) B[i] = A[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) B[0] += A[1 + a] * X[a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) B[0] += A[tmap2(a, b)] * X[a] * X[b] * 0.5;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) B[1 + a] += A[tmap2(a, b)] * X[b];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) B[0] += A[tmap3(a, b, c)] * X[a] * X[b] * X[c] * (1.0 / 6.0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) B[1 + a] += A[tmap3(a, b, c)] * X[b] * X[c] * 0.5;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = a; c < 3; ++c) B[tmap2(a, c)] += A[tmap3(a, b, c)] * X[b];
+}
+
+// parent cell holding coarse cell (ip, jp, kp) of a node in octant o
+// (node_server::compute_fmm, src/node_server.cpp:613-636, 756-768)
+__host__ __device__ __forceinline__ int parent_cell(int o, int ip, int jp, int kp) {
+    return (ip + 4 * (o & 1)) * 64 + (jp + 4 * ((o >> 1) & 1)) * 8 + (kp + 4 * ((o >> 2) & 1));
+}
+
+struct M2MArgs {
+    const double* M;        // [n][512][20] this level
+    double* com;            // [n][512][3]  this level (leaf centres written for RHO)
+    const double* mon;      // [n][512]
+    const int32_t* kind;
+    const int32_t* parent;
+    const int32_t* octant;
+    const double* xbase;    // [n][3]
+    double* pM;             // parent level [np][512][20]
+    double* pcom;           // parent level [np][512][3]
+    double dx;
+    int32_t nnodes;
+    int32_t rho;
+};
+
+struct L2LArgs {
+    double* L;              // [n][512][20] this level
+    double* Lc;             // [n][512][3]
+    const double* com;      // [n][512][3]
+    const int32_t* kind;
+    const int32_t* parent;
+    const int32_t* octant;
+    const double* pL;       // parent level
+    const double* pLc;
+    const double* pcom;
+    double* g;              // [n][512][4] or null
+    double G;
+    int32_t nnodes;
+    int32_t rho;
+    int32_t correct_am;
+};
+
 }   // namespace octofmm

@@ -586,6 +586,146 @@ __global__ void zero_fill(double* p, size_t n) {
     if (i < n) p[i] = 0.0;
 }
 
+// geometric centre xbase + i*dx, rounded as written (grid_fmm.cpp:1669-1671)
+__device__ __forceinline__ double cell_centre(double xb, int i, double dx) {
+#pragma clang fp contract(off)
+    return xb + (double)i * dx;
+}
+
+// Upward pass of one level (grid::compute_multipoles, src/grid_fmm.cpp:1639-1803):
+// one thread per coarse cell (64 per node).  For RHO the leaf nodes' cell
+// centres are written first and every coarse cell's mass-weighted centre goes
+// to the parent; the 8 children's multipoles are shifted to it and summed
+// into the parent's cell.
+__global__ __launch_bounds__(256) void m2m_kernel(M2MArgs a) {
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    if (gid >= a.nnodes * 64) return;
+    const int n = gid >> 6, q = gid & 63;
+    const int ip = q >> 4, jp = (q >> 2) & 3, kp = q & 3;
+    const bool leaf = a.kind[n] == OCTOFMM_SLOT_LEAF;
+    double* c0 = a.com + (size_t)n * kCells * 3;
+    int u[8];
+#pragma unroll
+    for (int ci = 0; ci < 8; ++ci)
+        u[ci] = (2 * ip + (ci & 1)) * 64 + (2 * jp + ((ci >> 1) & 1)) * 8 + (2 * kp + ((ci >> 2) & 1));
+    if (leaf && a.rho) {
+        const double* xb = a.xbase + 3 * n;
+#pragma unroll
+        for (int ci = 0; ci < 8; ++ci) {
+            double* p = c0 + 3 * u[ci];
+            p[0] = cell_centre(xb[0], u[ci] >> 6, a.dx);
+            p[1] = cell_centre(xb[1], (u[ci] >> 3) & 7, a.dx);
+            p[2] = cell_centre(xb[2], u[ci] & 7, a.dx);
+        }
+    }
+    const int pn = a.parent[n];
+    if (pn < 0) return;
+    const int pc = parent_cell(a.octant[n], ip, jp, kp);
+    double* Yp = a.pcom + ((size_t)pn * kCells + pc) * 3;
+    double Y[3];
+    if (a.rho) {
+        double mtot = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int ci = 0; ci < 8; ++ci) {
+            const double m = leaf ? a.mon[(size_t)n * kCells + u[ci]] : a.M[((size_t)n * kCells + u[ci]) * 20];
+            const double* X = c0 + 3 * u[ci];
+            mtot += m;
+            s0 += X[0] * m;
+            s1 += X[1] * m;
+            s2 += X[2] * m;
+        }
+        Y[0] = s0 / mtot;
+        Y[1] = s1 / mtot;
+        Y[2] = s2 / mtot;
+        Yp[0] = Y[0];
+        Yp[1] = Y[1];
+        Yp[2] = Y[2];
+    } else {
+        Y[0] = Yp[0];
+        Y[1] = Yp[1];
+        Y[2] = Yp[2];
+    }
+    double MM[20];
+#pragma unroll
+    for (int j = 0; j < 20; ++j) MM[j] = 0.0;
+    for (int ci = 0; ci < 8; ++ci) {
+        double A[20], B[20];
+        if (leaf) {
+            A[0] = a.mon[(size_t)n * kCells + u[ci]];
+#pragma unroll
+            for (int j = 1; j < 20; ++j) A[j] = 0.0;
+        } else {
+            const double2* src = reinterpret_cast<const double2*>(a.M + ((size_t)n * kCells + u[ci]) * 20);
+#pragma unroll
+            for (int j = 0; j < 10; ++j) {
+                const double2 v = src[j];
+                A[2 * j] = v.x;
+                A[2 * j + 1] = v.y;
+            }
+        }
+        const double* X = c0 + 3 * u[ci];
+        const double dX[3] = {X[0] - Y[0], X[1] - Y[1], X[2] - Y[2]};
+        multipole_shift(A, dX, B);
+#pragma unroll
+        for (int j = 0; j < 20; ++j) MM[j] += B[j];
+    }
+    double* Mo = a.pM + ((size_t)pn * kCells + pc) * 20;
+#pragma unroll
+    for (int j = 0; j < 20; ++j) Mo[j] = MM[j];
+}
+
+// Downward pass of one level with the leaf force assembly
+// (grid::compute_expansions, src/grid_fmm.cpp:1524-1637): one thread per cell.
+__global__ __launch_bounds__(256) void l2l_kernel(L2LArgs a) {
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    if (gid >= a.nnodes * kCells) return;
+    const int n = gid >> 9, u = gid & 511;
+    double l[20], lc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 20; ++j) l[j] = 0.0;
+    const int pn = a.parent[n];
+    if (pn >= 0) {
+        const int pc = parent_cell(a.octant[n], u >> 7, (u >> 4) & 3, (u >> 1) & 3);
+        double A[20];
+        const double* src = a.pL + ((size_t)pn * kCells + pc) * 20;
+#pragma unroll
+        for (int j = 0; j < 20; ++j) A[j] = src[j];
+        const double* Y = a.pcom + ((size_t)pn * kCells + pc) * 3;
+        const double* X = a.com + ((size_t)n * kCells + u) * 3;
+        const double dX[3] = {X[0] - Y[0], X[1] - Y[1], X[2] - Y[2]};
+        local_shift(A, dX, l);
+        if (a.rho) {
+            const double* s = a.pLc + ((size_t)pn * kCells + pc) * 3;
+            lc[0] = s[0];
+            lc[1] = s[1];
+            lc[2] = s[2];
+        }
+    }
+    double* Lo = a.L + ((size_t)n * kCells + u) * 20;
+    double Lf[20];
+#pragma unroll
+    for (int j = 0; j < 20; ++j) {
+        Lf[j] = Lo[j] + l[j];
+        Lo[j] = Lf[j];
+    }
+    double C[3] = {0.0, 0.0, 0.0};
+    if (a.rho) {
+        double* Co = a.Lc + ((size_t)n * kCells + u) * 3;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            C[d] = Co[d] + lc[d];
+            Co[d] = C[d];
+        }
+    }
+    if (a.g && a.kind[n] == OCTOFMM_SLOT_LEAF) {
+        double* go = a.g + ((size_t)n * kCells + u) * 4;
+        go[0] = a.G * Lf[0];
+        if (a.rho)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) go[1 + d] = -a.G * Lf[1 + d] - (a.correct_am ? a.G * C[d] : 0.0);
+    }
+}
+
 }   // namespace
 
 // ===========================================================================
@@ -635,6 +775,8 @@ struct octofmm_plan {
     octofmm_plan_stats stats;
     int variant = 8;           // level-kernel variant (see kVariants)
     uint64_t* d_mixbits = nullptr;
+    bool root = false;             // root sub-grid: root stencil over slot 0
+    double* d_partial = nullptr;   // root: per-chunk partial expansions
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 };
@@ -1013,6 +1155,8 @@ static void plan_free(octofmm_plan* p) {
     (void)hipFree(p->d_foff);
     (void)hipFree(p->d_fxbase);
     (void)hipFree(p->d_items);
+    (void)hipFree(p->d_mixbits);
+    (void)hipFree(p->d_partial);
     delete p;
 }
 
@@ -1033,7 +1177,27 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         hipLaunchKernelGGL(stage_records, dim3((nthreads + 255) / 256), dim3(256), 0, st, sa);
         HIPC(c, hipGetLastError());
     }
-    if ((stage_mask & 2) && p->nitems > 0) {
+    if ((stage_mask & 2) && p->root) {
+        RootArgs ra;
+        ra.rec = p->d_rec;
+        ra.tab = c->d_tab;
+        ra.partial = p->d_partial;
+        ra.per_chunk = kCells / kRootChunks;
+        ra.nchunks = kRootChunks;
+        if (p->type == OCTOFMM_RHO)
+            hipLaunchKernelGGL(root_kernel<true>, dim3(kRootChunks), dim3(512), 0, st, ra);
+        else
+            hipLaunchKernelGGL(root_kernel<false>, dim3(kRootChunks), dim3(512), 0, st, ra);
+        HIPC(c, hipGetLastError());
+        RootReduceArgs rr;
+        rr.partial = p->d_partial;
+        rr.L = p->d_L;
+        rr.Lc = p->d_Lc;
+        rr.nchunks = kRootChunks;
+        rr.rho = p->type == OCTOFMM_RHO;
+        hipLaunchKernelGGL(root_reduce, dim3(1), dim3(512), 0, st, rr);
+        HIPC(c, hipGetLastError());
+    } else if ((stage_mask & 2) && p->nitems > 0) {
         LevelArgs la;
         la.rec = p->d_rec;
         la.com = p->d_com;
@@ -1144,6 +1308,9 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
         if (lv->neighbors[(size_t)s * 27 + 13] != s)
             return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "neighbour entry 13 of slot %d must be the slot", s);
     }
+    const bool root = (lv->flags & OCTOFMM_PLAN_ROOT) != 0;
+    if (root && (ns != 1 || lv->ntargets != 1 || lv->targets[0] != 0 || lv->kind[0] != OCTOFMM_SLOT_NONLEAF))
+        return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "a root plan takes exactly one non-leaf slot as its target");
     // work items, heaviest kinds first: non-leaf with leaf neighbours, non-leaf,
     // leaf with refined neighbours, pure p2p leaves
     std::vector<int2> buckets[4];
@@ -1151,7 +1318,18 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
     std::memset(&st, 0, sizeof(st));
     const Tables& T = c->tab;
     std::vector<char> seen(ns, 0);
-    for (int t = 0; t < lv->ntargets; ++t) {
+    if (root) {
+        // every pair of the root's cells except self and near offsets
+        for (int t = 0; t < kCells; ++t)
+            for (int q = 0; q < kCells; ++q) {
+                const int dx = (q >> 6) - (t >> 6), dy = ((q >> 3) & 7) - ((t >> 3) & 7), dz = (q & 7) - (t & 7);
+                const bool inbox = dx >= -5 && dx <= 5 && dy >= -5 && dy <= 5 && dz >= -5 && dz <= 5;
+                if (inbox && T.root_excluded[(dx + 5) * 121 + (dy + 5) * 11 + (dz + 5)]) continue;
+                st.far_pairs++;
+            }
+        st.n_m2l_items = 1;
+    }
+    for (int t = 0; t < (root ? 0 : lv->ntargets); ++t) {
         const int s = lv->targets[t];
         if (s < 0 || s >= ns) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad target %d", t);
         if (seen[s]) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "duplicate target slot %d", s);
@@ -1216,6 +1394,8 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
     p->d_mon = lv->d_mon;
     p->d_L = lv->d_L;
     p->d_Lc = lv->d_L_c;
+    p->root = root;
+    if (root) p->precision = OCTOFMM_PRECISION_FP64;
     p->stats = st;
     std::vector<int32_t> foff(3 * (size_t)ns, kInx);
     auto cleanup = [&](octofmm_status s, const char* what) {
@@ -1237,6 +1417,8 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
         (!items.empty() &&
             hipMemcpy(p->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess))
         return cleanup(OCTOFMM_ERR_HIP, "plan table upload failed");
+    if (root && hipMalloc((void**)&p->d_partial, sizeof(double) * kRootChunks * kCells * kRecDoubles) != hipSuccess)
+        return cleanup(OCTOFMM_ERR_ALLOCATION, "root partial buffer allocation failed");
     if (p->precision == OCTOFMM_PRECISION_MIXED) {
         // far pairs with |d|^2 > mixed_far_d2 (cell units) are evaluated in FP32
         std::vector<uint64_t> mb(T.ndelta > 0 ? T.ndelta : 1, 0);
@@ -1359,6 +1541,323 @@ octofmm_status octofmm_level_plan_destroy(octofmm_plan* p) {
     if (!p) return OCTOFMM_OK;
     (void)hipSetDevice(p->ctx->device);
     plan_free(p);
+    return OCTOFMM_OK;
+}
+
+
+}   // extern "C"
+
+// ---------------------------------------------------------------------------
+// Whole-tree solve
+// ---------------------------------------------------------------------------
+struct octofmm_tree_plan {
+    struct Level {
+        int32_t nnodes = 0;
+        double dx = 0.0;
+        double* M = nullptr;
+        double* com = nullptr;
+        const double* mon = nullptr;
+        double* L = nullptr;
+        double* Lc = nullptr;
+        double* g = nullptr;
+        int32_t* d_kind = nullptr;
+        int32_t* d_parent = nullptr;
+        int32_t* d_octant = nullptr;
+        double* d_xbase = nullptr;
+        octofmm_plan* plan = nullptr;
+    };
+    octofmm_context* ctx = nullptr;
+    std::vector<Level> lv;
+    int32_t type = 0, flags = 0, correct_am = 1;
+    double G = 1.0;
+    octofmm_tree_stats stats;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+void tree_free(octofmm_tree_plan* t) {
+    if (!t) return;
+    if (t->gexec) (void)hipGraphExecDestroy(t->gexec);
+    if (t->graph) (void)hipGraphDestroy(t->graph);
+    for (auto& l : t->lv) {
+        plan_free(l.plan);
+        (void)hipFree(l.d_kind);
+        (void)hipFree(l.d_parent);
+        (void)hipFree(l.d_octant);
+        (void)hipFree(l.d_xbase);
+    }
+    delete t;
+}
+
+// phase bit 0: upward pass; bit 1: clear outputs + interactions; bit 2: downward pass
+octofmm_status tree_launch(octofmm_tree_plan* t, hipStream_t st, int phases) {
+    octofmm_context* c = t->ctx;
+    const int nl = (int)t->lv.size();
+    const int rho = t->type == OCTOFMM_RHO;
+    if (phases & 1)
+        for (int l = nl - 1; l >= 0; --l) {
+            const auto& L = t->lv[l];
+            M2MArgs a;
+            a.M = L.M;
+            a.com = L.com;
+            a.mon = L.mon;
+            a.kind = L.d_kind;
+            a.parent = L.d_parent;
+            a.octant = L.d_octant;
+            a.xbase = L.d_xbase;
+            a.pM = l > 0 ? t->lv[l - 1].M : nullptr;
+            a.pcom = l > 0 ? t->lv[l - 1].com : nullptr;
+            a.dx = L.dx;
+            a.nnodes = L.nnodes;
+            a.rho = rho;
+            hipLaunchKernelGGL(m2m_kernel, dim3((L.nnodes * 64 + 255) / 256), dim3(256), 0, st, a);
+            HIPC(c, hipGetLastError());
+        }
+    if (phases & 2) {
+        for (int l = 0; l < nl; ++l) {
+            const auto& L = t->lv[l];
+            const size_t nL = (size_t)L.nnodes * kCells * 20, nC = (size_t)L.nnodes * kCells * 3;
+            hipLaunchKernelGGL(zero_fill, dim3((unsigned)((nL + 255) / 256)), dim3(256), 0, st, L.L, nL);
+            if (rho) hipLaunchKernelGGL(zero_fill, dim3((unsigned)((nC + 255) / 256)), dim3(256), 0, st, L.Lc, nC);
+            HIPC(c, hipGetLastError());
+        }
+        for (int l = 0; l < nl; ++l) {
+            const octofmm_status s = plan_launch(t->lv[l].plan, st, 3);
+            if (s != OCTOFMM_OK) return s;
+        }
+    }
+    if (phases & 4)
+        for (int l = 0; l < nl; ++l) {
+            const auto& L = t->lv[l];
+            L2LArgs a;
+            a.L = L.L;
+            a.Lc = L.Lc;
+            a.com = L.com;
+            a.kind = L.d_kind;
+            a.parent = L.d_parent;
+            a.octant = L.d_octant;
+            a.pL = l > 0 ? t->lv[l - 1].L : nullptr;
+            a.pLc = l > 0 ? t->lv[l - 1].Lc : nullptr;
+            a.pcom = l > 0 ? t->lv[l - 1].com : nullptr;
+            a.g = L.g;
+            a.G = t->G;
+            a.nnodes = L.nnodes;
+            a.rho = rho;
+            a.correct_am = t->correct_am;
+            hipLaunchKernelGGL(l2l_kernel, dim3((L.nnodes * kCells + 255) / 256), dim3(256), 0, st, a);
+            HIPC(c, hipGetLastError());
+        }
+    return OCTOFMM_OK;
+}
+
+}   // namespace
+
+extern "C" {
+
+octofmm_status octofmm_tree_plan_create(octofmm_context* c, const octofmm_tree* tr, octofmm_tree_plan** out) {
+    if (!c) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    if (!tr || !out || tr->nlevels <= 0 || !tr->levels) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad tree");
+    *out = nullptr;
+    if (tr->type != OCTOFMM_RHO && tr->type != OCTOFMM_DRHODT) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad type");
+    const bool rho = tr->type == OCTOFMM_RHO;
+    const int nl = tr->nlevels;
+    // structure: root alone on level 0, 8 children per non-leaf node, leaves at the bottom
+    for (int l = 0; l < nl; ++l) {
+        const octofmm_tree_level& L = tr->levels[l];
+        if (L.nnodes <= 0 || !L.neighbors || !L.kind || !L.parent || !L.octant || !L.xbase || !(L.dx > 0.0))
+            return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad description of level %d", l);
+        if (!L.d_M || !L.d_com || !L.d_mon || !L.d_L || (rho && !L.d_L_c))
+            return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "null device array on level %d", l);
+        if (l == 0 && (L.nnodes != 1 || L.parent[0] != -1))
+            return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "level 0 must hold the root alone, without a parent");
+        std::vector<int> children(l + 1 < nl ? tr->levels[l + 1].nnodes : 0, 0);
+        for (int n = 0; n < L.nnodes; ++n) {
+            if (L.kind[n] != OCTOFMM_SLOT_NONLEAF && L.kind[n] != OCTOFMM_SLOT_LEAF)
+                return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad kind of node %d on level %d", n, l);
+            if (l == nl - 1 && L.kind[n] == OCTOFMM_SLOT_NONLEAF)
+                return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "non-leaf node %d on the finest level", n);
+            if (L.octant[n] < 0 || L.octant[n] > 7)
+                return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad octant of node %d on level %d", n, l);
+        }
+        if (l > 0) {
+            const octofmm_tree_level& U = tr->levels[l - 1];
+            std::vector<uint8_t> used((size_t)U.nnodes * 8, 0);
+            for (int n = 0; n < L.nnodes; ++n) {
+                const int pn = L.parent[n];
+                if (pn < 0 || pn >= U.nnodes || U.kind[pn] != OCTOFMM_SLOT_NONLEAF)
+                    return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "node %d on level %d has no non-leaf parent", n, l);
+                uint8_t& u = used[(size_t)pn * 8 + L.octant[n]];
+                if (u) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "two children in one octant on level %d", l);
+                u = 1;
+            }
+            for (int pn = 0; pn < U.nnodes; ++pn) {
+                if (U.kind[pn] != OCTOFMM_SLOT_NONLEAF) continue;
+                for (int o = 0; o < 8; ++o)
+                    if (!used[(size_t)pn * 8 + o])
+                        return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "non-leaf node %d on level %d lacks child %d",
+                            pn, l - 1, o);
+            }
+        }
+    }
+    HIPC(c, hipSetDevice(c->device));
+    octofmm_tree_plan* t = new octofmm_tree_plan();
+    t->ctx = c;
+    t->type = tr->type;
+    t->flags = tr->flags;
+    t->G = tr->G;
+    t->correct_am = tr->correct_am;
+    std::memset(&t->stats, 0, sizeof(t->stats));
+    t->stats.nlevels = nl;
+    t->lv.resize(nl);
+    auto cleanup = [&](octofmm_status s, const char* what) {
+        tree_free(t);
+        return fail(c, s, "%s", what);
+    };
+    for (int l = 0; l < nl; ++l) {
+        const octofmm_tree_level& L = tr->levels[l];
+        auto& D = t->lv[l];
+        D.nnodes = L.nnodes;
+        D.dx = L.dx;
+        D.M = L.d_M;
+        D.com = L.d_com;
+        D.mon = L.d_mon;
+        D.L = L.d_L;
+        D.Lc = L.d_L_c;
+        D.g = L.d_g;
+        const size_t n = (size_t)L.nnodes;
+        if (hipMalloc((void**)&D.d_kind, sizeof(int32_t) * n) != hipSuccess ||
+            hipMalloc((void**)&D.d_parent, sizeof(int32_t) * n) != hipSuccess ||
+            hipMalloc((void**)&D.d_octant, sizeof(int32_t) * n) != hipSuccess ||
+            hipMalloc((void**)&D.d_xbase, sizeof(double) * 3 * n) != hipSuccess)
+            return cleanup(OCTOFMM_ERR_ALLOCATION, "tree table allocation failed");
+        if (hipMemcpy(D.d_kind, L.kind, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(D.d_parent, L.parent, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(D.d_octant, L.octant, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(D.d_xbase, L.xbase, sizeof(double) * 3 * n, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(OCTOFMM_ERR_HIP, "tree table upload failed");
+        t->stats.nodes += L.nnodes;
+        for (int i = 0; i < L.nnodes; ++i) t->stats.leaves += L.kind[i] == OCTOFMM_SLOT_LEAF;
+    }
+    // one upward pass first, so that autotuning times the interactions on real multipoles
+    {
+        const octofmm_status s = tree_launch(t, c->stream, 1);
+        if (s != OCTOFMM_OK) {
+            tree_free(t);
+            return s;
+        }
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(OCTOFMM_ERR_HIP, "upward pass failed");
+    }
+    for (int l = 0; l < nl; ++l) {
+        const octofmm_tree_level& L = tr->levels[l];
+        std::vector<int32_t> targets(L.nnodes);
+        for (int i = 0; i < L.nnodes; ++i) targets[i] = i;
+        octofmm_level lv;
+        std::memset(&lv, 0, sizeof(lv));
+        lv.nslots = L.nnodes;
+        lv.neighbors = L.neighbors;
+        lv.kind = L.kind;
+        lv.xbase = L.xbase;
+        lv.dx = L.dx;
+        lv.ntargets = L.nnodes;
+        lv.targets = targets.data();
+        lv.type = tr->type;
+        lv.precision = tr->precision;
+        lv.mixed_far_d2 = tr->mixed_far_d2;
+        lv.flags = tr->flags & OCTOFMM_PLAN_AUTOTUNE;
+        if (l == 0 && L.kind[0] == OCTOFMM_SLOT_NONLEAF) {
+            lv.flags = OCTOFMM_PLAN_ROOT;
+            lv.precision = OCTOFMM_PRECISION_FP64;
+        }
+        lv.d_M = L.d_M;
+        lv.d_com = L.d_com;
+        lv.d_mon = L.d_mon;
+        lv.d_L = L.d_L;
+        lv.d_L_c = rho ? L.d_L_c : nullptr;
+        const octofmm_status s = octofmm_level_plan_create(c, &lv, &t->lv[l].plan);
+        if (s != OCTOFMM_OK) {
+            const std::string msg = c->err;
+            tree_free(t);
+            return fail(c, s, "level %d: %s", l, msg.c_str());
+        }
+        const octofmm_plan_stats& ps = t->lv[l].plan->stats;
+        if (t->lv[l].plan->root)
+            t->stats.root_pairs += ps.far_pairs;
+        else
+            t->stats.far_pairs += ps.far_pairs;
+        t->stats.near_pairs += ps.near_pairs;
+        t->stats.p2p_pairs += ps.p2p_pairs;
+        t->stats.p2m_pairs += ps.p2m_pairs;
+        t->stats.flops += ps.flops;
+        t->stats.mixed_far_pairs += ps.mixed_far_pairs;
+    }
+    if (tr->flags & OCTOFMM_PLAN_USE_GRAPH) {
+        hipStream_t cs = c->stream;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return cleanup(OCTOFMM_ERR_HIP, "graph capture failed to start");
+        const octofmm_status ls = tree_launch(t, cs, 7);
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(cs, &g);
+        if (ls != OCTOFMM_OK || ee != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return cleanup(OCTOFMM_ERR_HIP, "graph capture failed");
+        }
+        t->graph = g;
+        if (hipGraphInstantiate(&t->gexec, g, nullptr, nullptr, 0) != hipSuccess)
+            return cleanup(OCTOFMM_ERR_HIP, "graph instantiation failed");
+    }
+    *out = t;
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_tree_plan_execute(octofmm_tree_plan* t, void* stream) {
+    if (!t) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    octofmm_context* c = t->ctx;
+    HIPC(c, hipSetDevice(c->device));
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (t->gexec) {
+        HIPC(c, hipGraphLaunch(t->gexec, st));
+        return OCTOFMM_OK;
+    }
+    return tree_launch(t, st, 7);
+}
+
+octofmm_status octofmm_tree_plan_stats(const octofmm_tree_plan* t, octofmm_tree_stats* s) {
+    if (!t || !s) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    *s = t->stats;
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_tree_plan_time_phases(octofmm_tree_plan* t, void* stream, int32_t iters, double* ms) {
+    if (!t || !ms || iters <= 0) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    octofmm_context* c = t->ctx;
+    HIPC(c, hipSetDevice(c->device));
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    HIPC(c, hipEventCreate(&e0));
+    HIPC(c, hipEventCreate(&e1));
+    for (int ph = 0; ph < 3; ++ph) {
+        HIPC(c, hipEventRecord(e0, st));
+        for (int i = 0; i < iters; ++i) {
+            const octofmm_status s = tree_launch(t, st, 1 << ph);
+            if (s != OCTOFMM_OK) return s;
+        }
+        HIPC(c, hipEventRecord(e1, st));
+        HIPC(c, hipEventSynchronize(e1));
+        float f = 0.f;
+        HIPC(c, hipEventElapsedTime(&f, e0, e1));
+        ms[ph] = (double)f / iters;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_tree_plan_destroy(octofmm_tree_plan* t) {
+    if (!t) return OCTOFMM_OK;
+    (void)hipSetDevice(t->ctx->device);
+    tree_free(t);
     return OCTOFMM_OK;
 }
 

@@ -94,6 +94,12 @@ class DeviceArray:
     def zero(self):
         check(rt().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
 
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"upload of {a.nbytes} bytes into a {self.nbytes}-byte buffer")
+        check(rt().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, H2D), "hipMemcpy H2D")
+
     def free(self):
         if self.ptr:
             rt().hipFree(self.ptr)
@@ -170,3 +176,35 @@ class LevelOnDevice:
     def free(self):
         for a in (self.M, self.com, self.mon, self.L, self.L_c):
             a.free()
+
+
+class TreeOnDevice:
+    """A tree.Tree on the GPU: per level the leaf masses (uploaded) and the
+    arrays the solve writes (M, com, L, L_c, g)."""
+
+    def __init__(self, tree):
+        self.tree = tree
+        self.levels = []
+        for lv in tree.levels:
+            n = lv.nnodes
+            mon = lv.mon if lv.mon is not None else np.zeros((n, 512))
+            d = {"M": DeviceArray((n, 512, 20)), "com": DeviceArray((n, 512, 3)),
+                 "mon": DeviceArray.from_numpy(np.ascontiguousarray(mon, dtype=np.float64)),
+                 "L": DeviceArray((n, 512, 20)), "L_c": DeviceArray((n, 512, 3)), "g": DeviceArray((n, 512, 4))}
+            for k in ("M", "com", "L", "L_c", "g"):
+                d[k].zero()
+            self.levels.append(d)
+
+    def addresses(self):
+        return [{k: v.addr for k, v in d.items()} for d in self.levels]
+
+    def plan(self, ctx, **kw):
+        return ctx.tree_plan(self.tree, self.addresses(), **kw)
+
+    def fetch(self, *keys):
+        return [{k: d[k].numpy() for k in keys} for d in self.levels]
+
+    def free(self):
+        for d in self.levels:
+            for a in d.values():
+                a.free()

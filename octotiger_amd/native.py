@@ -18,7 +18,8 @@ EXPORTED = [
     "octofmm_destroy", "octofmm_compute_multipole_interactions",
     "octofmm_compute_monopole_interactions", "octofmm_level_plan_create",
     "octofmm_level_plan_execute", "octofmm_level_plan_stats", "octofmm_level_plan_time_stages",
-    "octofmm_level_plan_destroy",
+    "octofmm_level_plan_destroy", "octofmm_tree_plan_create", "octofmm_tree_plan_execute",
+    "octofmm_tree_plan_stats", "octofmm_tree_plan_time_phases", "octofmm_tree_plan_destroy",
 ]
 
 _lib = None
@@ -54,6 +55,11 @@ def lib():
         L.octofmm_level_plan_stats.argtypes = [vp, ctypes.POINTER(_abi.PlanStats)]
         L.octofmm_level_plan_time_stages.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
         L.octofmm_level_plan_destroy.argtypes = [vp]
+        L.octofmm_tree_plan_create.argtypes = [vp, ctypes.POINTER(_abi.Tree), ctypes.POINTER(vp)]
+        L.octofmm_tree_plan_execute.argtypes = [vp, vp]
+        L.octofmm_tree_plan_stats.argtypes = [vp, ctypes.POINTER(_abi.TreeStats)]
+        L.octofmm_tree_plan_time_phases.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        L.octofmm_tree_plan_destroy.argtypes = [vp]
         for fn in EXPORTED:
             getattr(L, fn).restype = getattr(L, fn).restype or ctypes.c_int
         L.octofmm_status_string.restype = ctypes.c_char_p
@@ -107,6 +113,69 @@ class Context:
         (addresses as ints, e.g. torch tensor data_ptr())."""
         return LevelPlan(self, level, d_M, d_com, d_mon, d_L, d_L_c, type_, precision, mixed_far_d2,
                          flags, targets)
+
+    def tree_plan(self, tree, dev, **kw):
+        """Plan of the whole gravity solve over a tree.Tree (see TreePlan)."""
+        return TreePlan(self, tree, dev, **kw)
+
+
+class TreePlan:
+    """A whole-tree gravity solve (upward M2M, interactions of every level,
+    downward L2L and the leaf force assembly) over device arrays, one set per
+    level: dev[l] = dict of int addresses M, com, mon, L, L_c, g."""
+
+    def __init__(self, ctx, tree, dev, type_=_abi.RHO, precision=_abi.PRECISION_FP64, mixed_far_d2=4,
+                 flags=_abi.PLAN_USE_GRAPH, G=1.0, correct_am=True):
+        self.ctx = ctx
+        self._keep = []
+        levels = (_abi.TreeLevel * len(tree.levels))()
+        for i, lv in enumerate(tree.levels):
+            arrs = [np.ascontiguousarray(lv.neighbors, dtype=np.int32), np.ascontiguousarray(lv.kind, dtype=np.int32),
+                    np.ascontiguousarray(lv.parent, dtype=np.int32), np.ascontiguousarray(lv.octant, dtype=np.int32),
+                    np.ascontiguousarray(lv.xbase, dtype=np.float64)]
+            self._keep.append(arrs)
+            nb, kd, pa, oc, xb = arrs
+            t = levels[i]
+            t.nnodes = lv.nnodes
+            t.neighbors, t.kind, t.parent, t.octant = _abi.iptr(nb), _abi.iptr(kd), _abi.iptr(pa), _abi.iptr(oc)
+            t.xbase = _abi.dptr(xb)
+            t.dx = float(lv.dx)
+            d = dev[i]
+            t.d_M, t.d_com, t.d_mon, t.d_L, t.d_L_c = d["M"], d["com"], d["mon"], d["L"], d["L_c"]
+            t.d_g = d.get("g")
+        self._levels = levels
+        tr = _abi.Tree()
+        tr.nlevels = len(tree.levels)
+        tr.levels = ctypes.cast(levels, ctypes.POINTER(_abi.TreeLevel))
+        tr.type, tr.precision, tr.mixed_far_d2, tr.flags = type_, precision, mixed_far_d2, flags
+        tr.G = G
+        tr.correct_am = 1 if correct_am else 0
+        self._h = ctypes.c_void_p()
+        ctx._check(lib().octofmm_tree_plan_create(ctx._h, ctypes.byref(tr), ctypes.byref(self._h)))
+
+    def execute(self, stream=0):
+        self.ctx._check(lib().octofmm_tree_plan_execute(self._h, ctypes.c_void_p(stream)))
+
+    def stats(self):
+        s = _abi.TreeStats()
+        self.ctx._check(lib().octofmm_tree_plan_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _abi.TreeStats._fields_}
+
+    def time_phases(self, stream=0, iters=5):
+        ms = (ctypes.c_double * 3)()
+        self.ctx._check(lib().octofmm_tree_plan_time_phases(self._h, ctypes.c_void_p(stream), iters, ms))
+        return list(ms)
+
+    def close(self):
+        if self._h:
+            lib().octofmm_tree_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class LevelPlan:

@@ -278,3 +278,122 @@ def subgrid_view(level, slot, root=False, type_=_abi.RHO):
             nbd.M = _abi.dptr(arr(level.M[n]))
             nbd.com = _abi.dptr(arr(level.com[n]))
     return sg, keep
+
+
+# ---------------------------------------------------------------------------
+# Adaptive trees for the full gravity solve
+# ---------------------------------------------------------------------------
+def lane_emden(n=1.5, h=1.0e-3):
+    """Lane-Emden solution theta(xi) of index n by classical RK4 from the
+    series start theta = 1 - xi^2/6 + n xi^4/120; returns (xi, theta, xi_1)
+    with xi_1 the first zero (3.65375 for n = 1.5)."""
+    def f(xi, y):
+        th, dth = y
+        return np.array([dth, -(max(th, 0.0) ** n) - 2.0 * dth / xi])
+
+    xi = 1.0e-4
+    y = np.array([1.0 - xi * xi / 6.0 + n * xi ** 4 / 120.0, -xi / 3.0 + n * xi ** 3 / 30.0])
+    xs, ts = [0.0, xi], [1.0, y[0]]
+    while y[0] > 0.0:
+        k1 = f(xi, y)
+        k2 = f(xi + 0.5 * h, y + 0.5 * h * k1)
+        k3 = f(xi + 0.5 * h, y + 0.5 * h * k2)
+        k4 = f(xi + h, y + h * k3)
+        y_new = y + (h / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+        if y_new[0] <= 0.0:
+            xi1 = xi + h * y[0] / (y[0] - y_new[0])
+            xs.append(xi1)
+            ts.append(0.0)
+            break
+        xi += h
+        y = y_new
+        xs.append(xi)
+        ts.append(y[0])
+    return np.array(xs), np.array(ts), xs[-1]
+
+
+def polytrope_density(r, radius=0.3, n=1.5, rho_c=1.0, floor=1.0e-10):
+    xi_t, th_t, xi1 = _lane_emden_table(n)
+    xi = np.asarray(r, dtype=np.float64) * (xi1 / radius)
+    th = np.interp(xi, xi_t, th_t, right=0.0)
+    return np.maximum(rho_c * np.power(np.maximum(th, 0.0), n), floor * rho_c)
+
+
+_LE_CACHE = {}
+
+
+def _lane_emden_table(n):
+    if n not in _LE_CACHE:
+        _LE_CACHE[n] = lane_emden(n)
+    return _LE_CACHE[n]
+
+
+def _assign_leaf_masses(tree, density, seed, noise_sigma):
+    """Leaf cell masses rho(centre) dx^3 x LogNormal(0, noise_sigma); the
+    noise counter is the cell's global index in level-major, Morton node order."""
+    base = 0
+    for lv in tree.levels:
+        cen = tree.cell_centres(lv)
+        mon = np.zeros((lv.nnodes, 512))
+        leaf = lv.kind == _abi.SLOT_LEAF
+        if leaf.any():
+            rho = density(cen[leaf])
+            cnt = base + np.arange(lv.nnodes * 512, dtype=np.uint64).reshape(lv.nnodes, 512)[leaf]
+            noise = np.exp(noise_sigma * normal(seed, cnt)) if noise_sigma > 0 else 1.0
+            mon[leaf] = rho * lv.dx ** 3 * noise
+        lv.mon = np.ascontiguousarray(mon)
+        base += lv.nnodes * 512
+    return tree
+
+
+def polytrope_tree(max_level=5, mass_threshold=2.0e-6, radius=0.3, n=1.5, seed=SEED_BASE + 5,
+                   noise_sigma=0.01, name="polytrope-tree"):
+    """Adaptive tree over the unit box [-1/2, 1/2]^3 for an index-n polytrope
+    of the given radius (centre at the origin): a node is refined where one of
+    its cells holds more mass than mass_threshold (rho(centre) dx^3), up to
+    max_level, then properly nested.  Leaf cells carry rho dx^3 with seeded
+    LogNormal(0, noise_sigma) noise.  The star's rotation changes velocities
+    only, so the gravity state does not depend on it."""
+    from . import tree as _tree
+
+    dens = lambda x: polytrope_density(np.sqrt((x * x).sum(axis=-1)), radius=radius, n=n)
+
+    def refine(lev, cs, xb, dxl):
+        i = np.arange(8, dtype=np.float64) * dxl
+        g = np.stack(np.meshgrid(i, i, i, indexing="ij"), axis=-1).reshape(512, 3)
+        cen = xb[:, None, :] + g[None]
+        return (dens(cen) * dxl ** 3).max(axis=1) > mass_threshold
+
+    t = _tree.build_tree(refine, max_level, name=name)
+    _assign_leaf_masses(t, dens, seed, noise_sigma)
+    t.info = {"seed": seed, "max_level": max_level, "mass_threshold": mass_threshold, "radius": radius,
+              "polytrope_index": n, "noise_sigma": noise_sigma,
+              "nodes_per_level": [lv.nnodes for lv in t.levels],
+              "leaves_per_level": [lv.nleaves for lv in t.levels], "nodes": t.nnodes}
+    return t
+
+
+def config5(seed=SEED_BASE + 5):
+    """Full adaptive-tree solve: n = 1.5 polytrope of radius 0.3 refined on
+    cell mass > 2.5e-8 up to level 6 -- 10,697 sub-grids on levels 0..6
+    (levels 0..2 refine fully, leaves sit on levels 3..6)."""
+    return polytrope_tree(max_level=6, mass_threshold=2.5e-8, seed=seed, name="polytrope-tree-c5")
+
+
+def small_tree(seed=SEED_BASE + 50, max_level=2, name="small-tree"):
+    """A few dozen nodes over 3 levels with leaves at every level: the root is
+    refined, 2 of its children are refined (so level 2 has 16 nodes, leaves and
+    refined-neighbour boundaries at level 1).  Masses log-normal (sigma 1)."""
+    from . import tree as _tree
+
+    def refine(lev, cs, xb, dxl):
+        if lev == 0:
+            return np.ones(len(cs), bool)
+        key = cs[:, 0] * 4 + cs[:, 1] * 2 + cs[:, 2]
+        return (key == 0) | (key == 1) if lev == 1 else np.zeros(len(cs), bool)
+
+    t = _tree.build_tree(refine, max_level, name=name)
+    _assign_leaf_masses(t, lambda x: np.ones(x.shape[:-1]), seed, 1.0)
+    t.info = {"seed": seed, "nodes_per_level": [lv.nnodes for lv in t.levels],
+              "leaves_per_level": [lv.nleaves for lv in t.levels], "nodes": t.nnodes}
+    return t

@@ -155,3 +155,121 @@ def level_interactions(level, theta=0.5, type_=_abi.RHO, targets=None, nthreads=
     if st != 0:
         raise RuntimeError(f"oracle level_interactions failed ({st})")
     return L, L_c
+
+
+# ---------------------------------------------------------------------------
+# Tree passes and the whole adaptive-tree solve
+# ---------------------------------------------------------------------------
+def _tree_lib():
+    L = lib()
+    if not getattr(L, "_tree_ready", False):
+        L.oracle_multipole_shift.argtypes = [_dp, _dp, _dp]
+        L.oracle_local_shift.argtypes = [_dp, _dp, _dp]
+        L.oracle_m2m_level.argtypes = [ctypes.c_int32, _ip, _ip, _ip, _dp, ctypes.c_double, ctypes.c_int32,
+                                       _dp, _dp, _dp, _dp, _dp]
+        L.oracle_l2l_level.argtypes = [ctypes.c_int32, _ip, _ip, _ip, ctypes.c_int32, _dp, _dp, _dp, _dp,
+                                       _dp, _dp, ctypes.c_double, ctypes.c_int32, _dp]
+        L._tree_ready = True
+    return L
+
+
+def multipole_shift(A, X):
+    """taylor::operator>>= : the multipole A moved by X (old - new centre)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    B = np.zeros(20)
+    _tree_lib().oracle_multipole_shift(_abi.dptr(A), _abi.dptr(X), _abi.dptr(B))
+    return B
+
+
+def local_shift(A, X):
+    """taylor::operator<<= : the local expansion A re-expanded at +X."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    B = np.zeros(20)
+    _tree_lib().oracle_local_shift(_abi.dptr(A), _abi.dptr(X), _abi.dptr(B))
+    return B
+
+
+class _LevelView:
+    """What level_interactions reads, for one tree level."""
+
+    def __init__(self, lv, M, com, mon):
+        self.nslots = lv.nnodes
+        self.neighbors = np.ascontiguousarray(lv.neighbors, dtype=np.int32)
+        self.kind = np.ascontiguousarray(lv.kind, dtype=np.int32)
+        self.xbase = np.ascontiguousarray(lv.xbase, dtype=np.float64)
+        self.dx = lv.dx
+        self.targets = np.arange(lv.nnodes, dtype=np.int32)
+        self.M, self.com, self.mon = M, com, mon
+
+
+def tree_solve(tree, theta=0.5, type_=_abi.RHO, G=1.0, correct_am=True, nthreads=None, use_ref=False, coms=None):
+    """The full gravity solve over an octree.Tree, as node_server::compute_fmm
+    runs it level by level (src/node_server.cpp:589-798): M2M up the tree,
+    interactions on every level (the root with the root stencil), then L2L
+    down the tree and the leaf force assembly.  Returns per-level dicts with
+    M, com, L, L_c and g ([n, 512, 4] = phi, gx, gy, gz; leaves only).
+    DRHODT reuses the centres of mass of an RHO solve: pass its per-level
+    com arrays as coms."""
+    T = _tree_lib()
+    rho = type_ == _abi.RHO
+    out = []
+    for lv in tree.levels:
+        n = lv.nnodes
+        out.append({
+            "M": np.zeros((n, 512, 20)), "com": np.zeros((n, 512, 3)),
+            "mon": np.ascontiguousarray(lv.mon, dtype=np.float64),
+            "L": np.zeros((n, 512, 20)), "L_c": np.zeros((n, 512, 3)), "g": np.zeros((n, 512, 4)),
+        })
+    if coms is not None:
+        for o, c in zip(out, coms):
+            o["com"][...] = c
+    elif not rho:
+        raise ValueError("a DRHODT solve needs the centres of mass of an RHO solve (coms=)")
+    ints = [tuple(np.ascontiguousarray(a, dtype=np.int32) for a in (lv.kind, lv.parent, lv.octant))
+            for lv in tree.levels]
+    # upward pass, finest level first
+    for li in range(len(tree.levels) - 1, -1, -1):
+        lv, o = tree.levels[li], out[li]
+        kind, parent, octant = ints[li]
+        up = out[li - 1] if li > 0 else None
+        st = T.oracle_m2m_level(lv.nnodes, _abi.iptr(kind), _abi.iptr(parent), _abi.iptr(octant),
+                                _abi.dptr(np.ascontiguousarray(lv.xbase)), lv.dx, type_, _abi.dptr(o["M"]),
+                                _abi.dptr(o["com"]), _abi.dptr(o["mon"]),
+                                _abi.dptr(up["M"]) if up else None, _abi.dptr(up["com"]) if up else None)
+        if st != 0:
+            raise RuntimeError(f"oracle_m2m_level failed on level {li} ({st})")
+    # interactions on every level
+    for li, lv in enumerate(tree.levels):
+        o = out[li]
+        if li == 0 and lv.kind[0] == _abi.SLOT_NONLEAF:
+            sg = _abi.Subgrid()
+            sg.M = _abi.dptr(o["M"][0])
+            sg.com = _abi.dptr(o["com"][0])
+            sg.mon = _abi.dptr(o["mon"][0])
+            sg.dx = lv.dx
+            for k in range(3):
+                sg.xbase[k] = lv.xbase[0, k]
+            sg.type = type_
+            sg.use_root_stencil = 1
+            multipole_interactions(sg, theta, L=o["L"][0], L_c=o["L_c"][0], use_ref=use_ref)
+        else:
+            Lv, Lcv = level_interactions(_LevelView(lv, o["M"], o["com"], o["mon"]), theta, type_,
+                                         nthreads=nthreads, use_ref=use_ref)
+            o["L"] += Lv
+            if rho:
+                o["L_c"][...] = Lcv
+    # downward pass, root first
+    for li, lv in enumerate(tree.levels):
+        o = out[li]
+        kind, parent, octant = ints[li]
+        up = out[li - 1] if li > 0 else None
+        st = T.oracle_l2l_level(lv.nnodes, _abi.iptr(kind), _abi.iptr(parent), _abi.iptr(octant), type_,
+                                _abi.dptr(o["com"]), _abi.dptr(up["L"]) if up else None,
+                                _abi.dptr(up["L_c"]) if up else None, _abi.dptr(up["com"]) if up else None,
+                                _abi.dptr(o["L"]), _abi.dptr(o["L_c"]), G, 1 if correct_am else 0,
+                                _abi.dptr(o["g"]))
+        if st != 0:
+            raise RuntimeError(f"oracle_l2l_level failed on level {li} ({st})")
+    return out

@@ -845,3 +845,190 @@ extern "C" int oracle_level_interactions(double theta, int32_t nslots, const int
     }
     return status;
 }
+
+// ---------------------------------------------------------------------------
+// Tree passes either side of the interaction kernels: the upward M2M pass and
+// the downward L2L pass with the leaf force assembly, one octree level at a
+// time.  Nodes of a level are 8^3 sub-grids; node n's parent is a non-leaf
+// node of the level above and octant[n] (bit 0 x, bit 1 y, bit 2 z, as
+// geo::octant::get_side, octotiger/geometry.hpp:333-335) says which 4^3 block
+// of the parent's cells the node covers.
+// ---------------------------------------------------------------------------
+namespace {
+
+// taylor<4> accessors (octotiger/taylor.hpp, src/taylor.cpp:10-58)
+constexpr int kMap2[3][3] = {{4, 5, 6}, {5, 7, 8}, {6, 8, 9}};
+constexpr int kMap3[3][3][3] = {{{10, 11, 12}, {11, 13, 14}, {12, 14, 15}},
+    {{11, 13, 14}, {13, 16, 17}, {14, 17, 18}}, {{12, 14, 15}, {14, 17, 18}, {15, 18, 19}}};
+
+// taylor::operator>>= (octotiger/taylor.hpp:222-255): moves a multipole
+// from its expansion centre by X (X = old centre - new centre).
+void multipole_shift(const double A[20], const double X[3], double B[20]) {
+    for (int i = 0; i < 20; ++i) B[i] = A[i];
+    for (int a = 0; a < 3; ++a) B[1 + a] += A[0] * X[a];
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) {
+            B[kMap2[a][b]] += A[1 + a] * X[b] + X[a] * A[1 + b];
+            B[kMap2[a][b]] += A[0] * X[a] * X[b];
+        }
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b)
+            for (int c = b; c < 3; ++c) {
+                double& t = B[kMap3[a][b][c]];
+                t += A[kMap2[a][b]] * X[c] + A[kMap2[b][c]] * X[a] + A[kMap2[c][a]] * X[b];
+                t += A[1 + a] * X[b] * X[c] + A[1 + b] * X[c] * X[a] + A[1 + c] * X[a] * X[b];
+                t += A[0] * X[a] * X[b] * X[c];
+            }
+}
+
+// taylor::operator<<= (octotiger/taylor.hpp:263-311): re-expands a local
+// expansion about a point displaced by X (X = new centre - old centre).
+void local_shift(const double A[20], const double X[3], double B[20]) {
+    for (int i = 0; i < 20; ++i) B[i] = A[i];
+    for (int a = 0; a < 3; ++a) B[0] += A[1 + a] * X[a];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) B[0] += A[kMap2[a][b]] * X[a] * X[b] * 0.5;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) B[1 + a] += A[kMap2[a][b]] * X[b];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) B[0] += A[kMap3[a][b][c]] * X[a] * X[b] * X[c] * (1.0 / 6.0);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) B[1 + a] += A[kMap3[a][b][c]] * X[b] * X[c] * 0.5;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c = a; c < 3; ++c) B[kMap2[a][c]] += A[kMap3[a][b][c]] * X[b];
+}
+
+// parent cell that holds coarse cell (ip, jp, kp) of a node in octant o
+// (node_server::compute_fmm gather, src/node_server.cpp:613-636)
+inline int parent_cell(int o, int ip, int jp, int kp) {
+    return (ip + 4 * (o & 1)) * 64 + (jp + 4 * ((o >> 1) & 1)) * 8 + (kp + 4 * ((o >> 2) & 1));
+}
+
+inline int child_cell(int ip, int jp, int kp, int ci) {
+    return (2 * ip + (ci & 1)) * 64 + (2 * jp + ((ci >> 1) & 1)) * 8 + (2 * kp + ((ci >> 2) & 1));
+}
+
+}   // namespace
+
+extern "C" {
+
+void oracle_multipole_shift(const double* A, const double* X, double* B) { multipole_shift(A, X, B); }
+void oracle_local_shift(const double* A, const double* X, double* B) { local_shift(A, X, B); }
+
+// Upward pass of one level (grid::compute_multipoles, src/grid_fmm.cpp:1639-1803).
+// Leaf nodes (RHO): their cell centres com = xbase + i*dx are written first
+// (:1662-1675).  Every node with a parent then hands its 4^3 coarse
+// multipoles and centres of mass to the parent's cells (:1696-1790); DRHODT
+// keeps the centres of mass of the RHO pass (:1700-1723 run for RHO only).
+int oracle_m2m_level(int32_t nnodes, const int32_t* kind, const int32_t* parent, const int32_t* octant,
+    const double* xbase, double dx, int32_t type, const double* M, double* com, const double* mon,
+    double* parent_M, double* parent_com) {
+    if (nnodes <= 0 || !kind || !parent || !octant || !xbase || !com) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    const bool rho = type == OCTOFMM_RHO;
+    for (int n = 0; n < nnodes; ++n) {
+        const bool leaf = kind[n] == OCTOFMM_SLOT_LEAF;
+        if (leaf && !mon) return OCTOFMM_ERR_INVALID_ARGUMENT;
+        if (!leaf && !M) return OCTOFMM_ERR_INVALID_ARGUMENT;
+        double* c0 = com + (size_t)n * NCELL * 3;
+        if (leaf && rho)
+            for (int i = 0; i < INX; ++i)
+                for (int j = 0; j < INX; ++j)
+                    for (int k = 0; k < INX; ++k) {
+                        double* p = c0 + 3 * inner_flat(i, j, k);
+                        p[0] = xbase[3 * n + 0] + i * dx;
+                        p[1] = xbase[3 * n + 1] + j * dx;
+                        p[2] = xbase[3 * n + 2] + k * dx;
+                    }
+        const int pn = parent[n];
+        if (pn < 0) continue;
+        if (!parent_M || !parent_com) return OCTOFMM_ERR_INVALID_ARGUMENT;
+        for (int ip = 0; ip < 4; ++ip)
+            for (int jp = 0; jp < 4; ++jp)
+                for (int kp = 0; kp < 4; ++kp) {
+                    const int pc = parent_cell(octant[n], ip, jp, kp);
+                    double* Y = parent_com + ((size_t)pn * NCELL + pc) * 3;
+                    if (rho) {
+                        double mc[8], X[3][8];
+                        for (int ci = 0; ci < 8; ++ci) {
+                            const int u = child_cell(ip, jp, kp, ci);
+                            mc[ci] = leaf ? mon[(size_t)n * NCELL + u] : M[((size_t)n * NCELL + u) * 20];
+                            for (int d = 0; d < 3; ++d) X[d][ci] = c0[3 * u + d];
+                        }
+                        double mtot = 0.0;
+                        for (int ci = 0; ci < 8; ++ci) mtot += mc[ci];
+                        for (int d = 0; d < 3; ++d) {
+                            double s = 0.0;
+                            for (int ci = 0; ci < 8; ++ci) s += X[d][ci] * mc[ci];
+                            Y[d] = s / mtot;
+                        }
+                    }
+                    double MM[20] = {0.0};
+                    for (int ci = 0; ci < 8; ++ci) {
+                        const int u = child_cell(ip, jp, kp, ci);
+                        double A[20] = {0.0}, B[20];
+                        if (leaf)
+                            A[0] = mon[(size_t)n * NCELL + u];
+                        else
+                            for (int j = 0; j < 20; ++j) A[j] = M[((size_t)n * NCELL + u) * 20 + j];
+                        const double dX[3] = {c0[3 * u] - Y[0], c0[3 * u + 1] - Y[1], c0[3 * u + 2] - Y[2]};
+                        multipole_shift(A, dX, B);
+                        for (int j = 0; j < 20; ++j) MM[j] += B[j];
+                    }
+                    double* Mo = parent_M + ((size_t)pn * NCELL + pc) * 20;
+                    for (int j = 0; j < 20; ++j) Mo[j] = MM[j];
+                }
+    }
+    return OCTOFMM_OK;
+}
+
+// Downward pass of one level (grid::compute_expansions, src/grid_fmm.cpp:1524-1637,
+// fed as node_server::compute_fmm does, src/node_server.cpp:738-772): the
+// parent's final expansion of the cell above is re-expanded about each child
+// cell's centre of mass and added to L, the parent's L_c is added unchanged,
+// and leaf nodes assemble phi = G L0 and g = -G (L_1..3 + L_c) into
+// g[node][cell][4] (DRHODT: phi slot only).  Nodes without a parent add zeros.
+int oracle_l2l_level(int32_t nnodes, const int32_t* kind, const int32_t* parent, const int32_t* octant,
+    int32_t type, const double* com, const double* parent_L, const double* parent_Lc, const double* parent_com,
+    double* L, double* L_c, double G, int32_t correct_am, double* g) {
+    if (nnodes <= 0 || !kind || !parent || !octant || !com || !L) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    const bool rho = type == OCTOFMM_RHO;
+    if (rho && !L_c) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    for (int n = 0; n < nnodes; ++n) {
+        const int pn = parent[n];
+        if (pn >= 0 && (!parent_L || !parent_com || (rho && !parent_Lc))) return OCTOFMM_ERR_INVALID_ARGUMENT;
+        for (int u = 0; u < NCELL; ++u) {
+            const int i = u / 64, j = (u / 8) % 8, k = u % 8;
+            double l[20] = {0.0}, lc[3] = {0.0, 0.0, 0.0};
+            if (pn >= 0) {
+                const int pc = parent_cell(octant[n], i >> 1, j >> 1, k >> 1);
+                const double* A = parent_L + ((size_t)pn * NCELL + pc) * 20;
+                const double* Y = parent_com + ((size_t)pn * NCELL + pc) * 3;
+                const double* X = com + ((size_t)n * NCELL + u) * 3;
+                const double dX[3] = {X[0] - Y[0], X[1] - Y[1], X[2] - Y[2]};
+                local_shift(A, dX, l);
+                if (rho)
+                    for (int d = 0; d < 3; ++d) lc[d] = parent_Lc[((size_t)pn * NCELL + pc) * 3 + d];
+            }
+            double* Lo = L + ((size_t)n * NCELL + u) * 20;
+            for (int c = 0; c < 20; ++c) Lo[c] += l[c];
+            double* Co = rho ? L_c + ((size_t)n * NCELL + u) * 3 : nullptr;
+            if (rho)
+                for (int d = 0; d < 3; ++d) Co[d] += lc[d];
+            if (g && kind[n] == OCTOFMM_SLOT_LEAF) {
+                double* go = g + ((size_t)n * NCELL + u) * 4;
+                go[0] = G * Lo[0];
+                if (rho)
+                    for (int d = 0; d < 3; ++d) {
+                        go[1 + d] = -G * Lo[1 + d];
+                        if (correct_am) go[1 + d] -= G * Co[d];
+                    }
+            }
+        }
+    }
+    return OCTOFMM_OK;
+}
+
+}   // extern "C"
