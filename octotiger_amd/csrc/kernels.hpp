@@ -393,8 +393,15 @@ struct LevelArgs {
     const int2* items;      // (slot, flags)
     const Tables* tab;
     const uint64_t* mixbits;   // per coarse offset: byte k, bit j = far pair evaluated in FP32
+    const double* mon;      // [nslots][512] monopoles (p2p window of leaf targets)
+    // p2m pair lists by neighbour direction (raw 0..26): dir[2r] targets in
+    // direction r starting at dir[2r+1] of tgt; tgt = (target cell, first
+    // partner, partner count, 0); part = partner record index in the neighbour
+    const int32_t* p2m_dir;
+    const int4* p2m_tgt;
+    const int32_t* p2m_part;
     int32_t nitems;
-    int32_t pad;
+    int32_t item0;          // first item of this launch
     double inv_dx;
     double neg_inv_dx;
 };

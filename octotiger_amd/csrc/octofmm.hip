@@ -426,52 +426,123 @@ __device__ void m2l_item_classwave(const LevelArgs& a, int slot, int flags, cons
     }
 }
 
+// Leaf targets (monopole kernels): one 512-lane workgroup per leaf sub-grid.
+// The monopoles of the sub-grid and of its leaf neighbours within the stencil
+// reach are staged once into an LDS window (zeros for refined or missing
+// neighbours, whose cells the monopole kernel does not see).  Wave w takes
+// child class k = w over the 64 coarse cells, so the partner offsets of its
+// p2p loop -- and the four constants (-1/r, d/r^3) per offset -- are the same
+// for every lane: each pair is one LDS read and four FMAs with scalar
+// operands (monopole_kernel_templates.hpp:16-23, kokkos_kernel.hpp:255-284;
+// the 1/dx and -1/dx^2 factors are applied once per cell).  Refined
+// neighbours then contribute through p2m with their full multipoles over the
+// same stencil, far and near alike (monopole_kernel_templates.hpp:69-239).
 template <bool RHO>
-__device__ void leaf_item(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind, int w) {
-    const LaneGeom g = lane_geom(w);
-    const Tables* __restrict__ tab = a.tab;
-    const double* xc = a.com + ((size_t)slot * kCells + g.cell) * 3;
-    const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
-    const bool p2m = (flags & kItemP2M) != 0;
-    double L[4] = {0.0, 0.0, 0.0, 0.0}, C[3] = {0.0, 0.0, 0.0};
-    const int kx = (g.k >> 2) & 1, ky = (g.k >> 1) & 1, kz = g.k & 1;
-    const int nd = tab->ndelta;
-    for (int di = 0; di < nd; ++di) {
-        const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
-        unsigned bits = (unsigned)((tab->farbits[di] | tab->nearbits[di]) >> (8 * g.k)) & 0xffu;
-        int ns, raw;
-        const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
-        const int nk = ns < 0 ? -1 : s_kind[raw];
-        if (ns < 0 || (nk != OCTOFMM_SLOT_LEAF && !p2m)) bits = 0;
-        const double* pb = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
-#pragma unroll 1
-        for (int j = 0; j < 8; ++j) {
-            if (!(bits & (1u << j))) continue;
-            const double* p = pb + j * kRecDoubles;
-            if (nk == OCTOFMM_SLOT_LEAF) {
-                // p2p (monopole_kernel_templates.hpp:16-23, kokkos_kernel.hpp:255-284)
-                const int dx = 2 * ax + ((j >> 2) & 1) - kx, dy = 2 * ay + ((j >> 1) & 1) - ky,
-                          dz = 2 * az + (j & 1) - kz;
-                const double* f = tab->four[(dx + 5) * 121 + (dy + 5) * 11 + (dz + 5)];
-                const double m = p[kRecMon] * a.inv_dx;
-                L[0] += f[0] * m;
-                L[1] += f[1] * m * a.neg_inv_dx;
-                L[2] += f[2] * m * a.neg_inv_dx;
-                L[3] += f[3] * m * a.neg_inv_dx;
-            } else {
-                p2m_pair<RHO>(p, X0, X1, X2, L, C);
-            }
-        }
+__global__ __launch_bounds__(512) void leaf_kernel(LevelArgs a) {
+    extern __shared__ double s_win[];
+    __shared__ int s_nbr[27];
+    __shared__ int s_kind[27];
+    const int2 it = a.items[a.item0 + xcd_item(blockIdx.x, gridDim.x)];
+    const int slot = it.x;
+    if (threadIdx.x < 27) {
+        const int n = a.nbr[(size_t)slot * 27 + threadIdx.x];
+        s_nbr[threadIdx.x] = n;
+        s_kind[threadIdx.x] = n >= 0 ? a.kind[n] : -1;
     }
-    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
+    __syncthreads();
+    const Tables* __restrict__ tab = a.tab;
+    const int h = tab->p2p_halo, W = kInx + 2 * h;
+    for (int i = threadIdx.x; i < W * W * W; i += 512) {
+        const int x = i / (W * W) - h, y = (i / W) % W - h, z = i % W - h;
+        const int raw = (x < 0 ? 0 : x < kInx ? 1 : 2) + 3 * (y < 0 ? 0 : y < kInx ? 1 : 2) +
+            9 * (z < 0 ? 0 : z < kInx ? 1 : 2);
+        const int ns = s_nbr[raw];
+        double m = 0.0;
+        if (ns >= 0 && s_kind[raw] == OCTOFMM_SLOT_LEAF)
+            m = a.mon[(size_t)ns * kCells + ((x + kInx) & 7) * 64 + ((y + kInx) & 7) * 8 + ((z + kInx) & 7)];
+        s_win[i] = m;
+    }
+    __syncthreads();
+    const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int l = threadIdx.x & 63;
+    const int Cx = l >> 4, Cy = (l >> 2) & 3, Cz = l & 3;
+    const int tx = 2 * Cx + ((k >> 2) & 1), ty = 2 * Cy + ((k >> 1) & 1), tz = 2 * Cz + (k & 1);
+    const int cell = tx * 64 + ty * 8 + tz;
+    const double* wp = s_win + ((tx + h) * W + (ty + h)) * W + (tz + h) - 2048;
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    const int32_t* __restrict__ list = tab->p2plist[k];
+    const int n = tab->np2p_k[k];
+#pragma unroll 4
+    for (int q = 0; q < n; ++q) {
+        const int e = list[q];
+        const double* f = tab->four[e & 0xffff];
+        const double m = wp[e >> 16];
+        p0 = fma(f[0], m, p0);
+        p1 = fma(f[1], m, p1);
+        p2 = fma(f[2], m, p2);
+        p3 = fma(f[3], m, p3);
+    }
+    const double s1 = a.inv_dx * a.neg_inv_dx;
+    double L[4] = {p0 * a.inv_dx, p1 * s1, p2 * s1, p3 * s1};
+    double C[3] = {0.0, 0.0, 0.0};
+    const bool p2m = (it.y & kItemP2M) != 0;
+    if (p2m) {
+        // refined neighbours, one direction at a time: each listed target of
+        // the direction is one thread, its sums go to the target's LDS slot;
+        // a barrier separates directions, so every target's sum is formed in
+        // the same order on every run
+        double* acc = s_win;   // the p2p window is no longer needed
+        __syncthreads();
+        for (int i = threadIdx.x; i < kCells * 7; i += 512) acc[i] = 0.0;
+        __syncthreads();
+        for (int r = 0; r < 27; ++r) {
+            const int ns = s_nbr[r];
+            if (r == 13 || ns < 0 || s_kind[r] != OCTOFMM_SLOT_NONLEAF) continue;
+            const int nt = a.p2m_dir[2 * r], t0 = a.p2m_dir[2 * r + 1];
+            for (int q = threadIdx.x; q < nt; q += 512) {
+                const int4 te = a.p2m_tgt[t0 + q];
+                const double* xc = a.com + ((size_t)slot * kCells + te.x) * 3;
+                const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
+                double Lp[4] = {0.0, 0.0, 0.0, 0.0}, Cp[3] = {0.0, 0.0, 0.0};
+                const double* base = a.rec + (size_t)ns * kCells * kRecDoubles;
+                for (int e = 0; e < te.z; ++e)
+                    p2m_pair<RHO>(base + (size_t)a.p2m_part[te.y + e] * kRecDoubles, X0, X1, X2, Lp, Cp);
+                double* o = acc + te.x * 7;
+                o[0] += Lp[0];
+                o[1] += Lp[1];
+                o[2] += Lp[2];
+                o[3] += Lp[3];
+                o[4] += Cp[0];
+                o[5] += Cp[1];
+                o[6] += Cp[2];
+            }
+            __syncthreads();
+        }
+        const double* o = acc + cell * 7;
+        L[0] += o[0];
+        L[1] += o[1];
+        L[2] += o[2];
+        L[3] += o[3];
+        C[0] = o[4];
+        C[1] = o[5];
+        C[2] = o[6];
+    }
+    double* Lo = a.L + ((size_t)slot * kCells + cell) * 20;
 #pragma unroll
     for (int i = 0; i < 4; ++i) Lo[i] += L[i];
     if (RHO && p2m) {
-        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
+        double* Co = a.Lc + ((size_t)slot * kCells + cell) * 3;
         Co[0] = C[0];
         Co[1] = C[1];
         Co[2] = C[2];
     }
+}
+
+template <bool RHO>
+void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, int halo) {
+    const int W = kInx + 2 * halo;
+    const size_t lds = sizeof(double) * std::max<size_t>((size_t)W * W * W, (size_t)kCells * 7);
+    hipLaunchKernelGGL(leaf_kernel<RHO>, dim3(nitems), dim3(512), lds, st, la);
 }
 
 // One target sub-grid is split over 8 / WAVES workgroups of WAVES waves.  The
@@ -486,7 +557,7 @@ __global__ __launch_bounds__(64 * WAVES) void level_kernel(LevelArgs a) {
     const int v = xcd_item(blockIdx.x, gridDim.x);
     const int item = v / kParts;
     const int w = (v % kParts) * WAVES + (threadIdx.x >> 6);
-    const int2 it = a.items[item];
+    const int2 it = a.items[a.item0 + item];
     if (threadIdx.x < 27) {
         const int n = a.nbr[(size_t)it.x * 27 + threadIdx.x];
         s_nbr[threadIdx.x] = n;
@@ -494,8 +565,8 @@ __global__ __launch_bounds__(64 * WAVES) void level_kernel(LevelArgs a) {
     }
     __syncthreads();
     if (it.y & kItemLeaf)
-        leaf_item<RHO>(a, it.x, it.y, s_nbr, s_kind, w);
-    else if (MAPPING == 1)
+        return;   // leaf items run in leaf_kernel
+    if (MAPPING == 1)
         m2l_item_classwave<RHO>(a, it.x, it.y, s_nbr, s_kind, w);
     else if (MAPPING == 2)
         m2l_item_lds<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage + (threadIdx.x >> 6) * kStageWave);
@@ -561,23 +632,18 @@ __global__ __launch_bounds__(512) void root_kernel(RootArgs a) {
     o[22] = C[2];
 }
 
-__global__ __launch_bounds__(512) void root_reduce(RootReduceArgs a) {
-    const int t = threadIdx.x;
-    double s[23];
-#pragma unroll
-    for (int i = 0; i < 23; ++i) s[i] = 0.0;
-    for (int q = 0; q < a.nchunks; ++q) {
-        const double* p = a.partial + ((size_t)q * kCells + t) * kRecDoubles;
-#pragma unroll
-        for (int i = 0; i < 23; ++i) s[i] += p[i];
-    }
-#pragma unroll
-    for (int i = 0; i < 20; ++i) a.L[(size_t)t * 20 + i] += s[i];
-    if (a.rho) {
-        a.Lc[(size_t)t * 3 + 0] = s[20];
-        a.Lc[(size_t)t * 3 + 1] = s[21];
-        a.Lc[(size_t)t * 3 + 2] = s[22];
-    }
+// One thread per output value (cell, component); the 64 chunks are summed in
+// chunk order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void root_reduce(RootReduceArgs a) {
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    if (gid >= kCells * 23) return;
+    const int t = gid / 23, c = gid - 23 * (gid / 23);
+    double s = 0.0;
+    for (int q = 0; q < a.nchunks; ++q) s += a.partial[((size_t)q * kCells + t) * kRecDoubles + c];
+    if (c < 20)
+        a.L[(size_t)t * 20 + c] += s;
+    else if (a.rho)
+        a.Lc[(size_t)t * 3 + (c - 20)] = s;
 }
 
 // Fill kernel used instead of hipMemsetAsync so the passes stay graph-safe.
@@ -755,11 +821,16 @@ struct octofmm_context {
     int32_t* d_meta = nullptr;
     double* d_fx = nullptr;
     double* d_partial = nullptr;   // root chunks
+    // p2m pair lists by direction (built from the tables at creation)
+    int32_t* d_p2m_dir = nullptr;
+    int4* d_p2m_tgt = nullptr;
+    int32_t* d_p2m_part = nullptr;
 };
 
 struct octofmm_plan {
     octofmm_context* ctx = nullptr;
     int32_t nslots = 0, nitems = 0, type = 0, precision = 0, flags = 0;
+    int32_t n_m2l = 0;             // items [0, n_m2l) are non-leaf, the rest leaf targets
     double dx = 0.0;
     const double* d_M = nullptr;
     const double* d_com = nullptr;
@@ -916,6 +987,44 @@ octofmm_status fill_workspace(octofmm_context* c, const octofmm_subgrid* in, boo
     return OCTOFMM_OK;
 }
 
+// p2m pair lists: for each neighbour direction, the target cells whose
+// stencil reaches into it and, per target, the partner records there, in
+// (coarse offset, child) order.
+void build_p2m_lists(const Tables& T, std::vector<int32_t>& dir, std::vector<int4>& tgt,
+    std::vector<int32_t>& part) {
+    std::vector<int32_t> lists[27][kCells];
+    for (int t = 0; t < kCells; ++t) {
+        const int x = t >> 6, y = (t >> 3) & 7, z = t & 7;
+        const int k = child_of(x, y, z);
+        for (int half = 0; half < 2; ++half) {
+            const uint16_t* pl = half ? T.nearlist[k] : T.farlist[k];
+            const int np = half ? T.nnear_k[k] : T.nfar_k[k];
+            for (int q = 0; q < np; ++q) {
+                const int di = pl[q] >> 3, j = pl[q] & 7;
+                const int px = 2 * ((x >> 1) + T.delta[di][0]) + ((j >> 2) & 1);
+                const int py = 2 * ((y >> 1) + T.delta[di][1]) + ((j >> 1) & 1);
+                const int pz = 2 * ((z >> 1) + T.delta[di][2]) + (j & 1);
+                const int r = (px < 0 ? 0 : px < kInx ? 1 : 2) + 3 * (py < 0 ? 0 : py < kInx ? 1 : 2) +
+                    9 * (pz < 0 ? 0 : pz < kInx ? 1 : 2);
+                if (r == 13) continue;
+                lists[r][t].push_back(record_of_cell(px & 7, py & 7, pz & 7));
+            }
+        }
+    }
+    dir.assign(54, 0);
+    tgt.clear();
+    part.clear();
+    for (int r = 0; r < 27; ++r) {
+        dir[2 * r + 1] = (int32_t)tgt.size();
+        for (int t = 0; t < kCells; ++t) {
+            if (lists[r][t].empty()) continue;
+            tgt.push_back(make_int4(t, (int)part.size(), (int)lists[r][t].size(), 0));
+            part.insert(part.end(), lists[r][t].begin(), lists[r][t].end());
+        }
+        dir[2 * r] = (int32_t)tgt.size() - dir[2 * r + 1];
+    }
+}
+
 // Uploads the staged 27-slot level and its tables.
 octofmm_status upload_workspace(octofmm_context* c, int item_flags) {
     int32_t* meta = c->h_meta;
@@ -983,7 +1092,7 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         rr.Lc = C13;
         rr.nchunks = kRootChunks;
         rr.rho = rho;
-        hipLaunchKernelGGL(root_reduce, dim3(1), dim3(512), 0, st, rr);
+        hipLaunchKernelGGL(root_reduce, dim3((kCells * 23 + 255) / 256), dim3(256), 0, st, rr);
         HIPC(c, hipGetLastError());
     } else {
         LevelArgs la;
@@ -996,13 +1105,24 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         la.items = reinterpret_cast<const int2*>(c->d_meta + kMetaItems);
         la.tab = c->d_tab;
         la.mixbits = nullptr;
+        la.mon = c->d_mon;
+        la.p2m_dir = c->d_p2m_dir;
+        la.p2m_tgt = c->d_p2m_tgt;
+        la.p2m_part = c->d_p2m_part;
         la.nitems = 1;
+        la.item0 = 0;
         la.inv_dx = 1.0 / dx;
         la.neg_inv_dx = -1.0 / dx;
-        if (rho)
+        if (item_flags & kItemLeaf) {
+            if (rho)
+                launch_leaf<true>(1, st, la, c->tab.p2p_halo);
+            else
+                launch_leaf<false>(1, st, la, c->tab.p2p_halo);
+        } else if (rho) {
             launch_level<true>(8, 1, st, la);
-        else
+        } else {
             launch_level<false>(8, 1, st, la);
+        }
         HIPC(c, hipGetLastError());
     }
     // results: L (20) then L_c (3) per cell into h_out
@@ -1055,6 +1175,20 @@ octofmm_status octofmm_create(const octofmm_config* cfg, octofmm_context** out) 
         octofmm_destroy(c);
         return OCTOFMM_ERR_HIP;
     }
+    std::vector<int32_t> pdir, ppart;
+    std::vector<int4> ptgt;
+    build_p2m_lists(c->tab, pdir, ptgt, ppart);
+    if (hipMalloc((void**)&c->d_p2m_dir, sizeof(int32_t) * pdir.size()) != hipSuccess ||
+        hipMalloc((void**)&c->d_p2m_tgt, sizeof(int4) * std::max<size_t>(1, ptgt.size())) != hipSuccess ||
+        hipMalloc((void**)&c->d_p2m_part, sizeof(int32_t) * std::max<size_t>(1, ppart.size())) != hipSuccess ||
+        hipMemcpy(c->d_p2m_dir, pdir.data(), sizeof(int32_t) * pdir.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        (!ptgt.empty() &&
+            hipMemcpy(c->d_p2m_tgt, ptgt.data(), sizeof(int4) * ptgt.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+        (!ppart.empty() &&
+            hipMemcpy(c->d_p2m_part, ppart.data(), sizeof(int32_t) * ppart.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+        octofmm_destroy(c);
+        return OCTOFMM_ERR_HIP;
+    }
     *out = c;
     return OCTOFMM_OK;
 }
@@ -1079,6 +1213,9 @@ octofmm_status octofmm_destroy(octofmm_context* c) {
     (void)hipFree(c->d_fx);
     (void)hipFree(c->d_partial);
     (void)hipFree(c->d_tab);
+    (void)hipFree(c->d_p2m_dir);
+    (void)hipFree(c->d_p2m_tgt);
+    (void)hipFree(c->d_p2m_part);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OCTOFMM_OK;
@@ -1195,7 +1332,7 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         rr.Lc = p->d_Lc;
         rr.nchunks = kRootChunks;
         rr.rho = p->type == OCTOFMM_RHO;
-        hipLaunchKernelGGL(root_reduce, dim3(1), dim3(512), 0, st, rr);
+        hipLaunchKernelGGL(root_reduce, dim3((kCells * 23 + 255) / 256), dim3(256), 0, st, rr);
         HIPC(c, hipGetLastError());
     } else if ((stage_mask & 2) && p->nitems > 0) {
         LevelArgs la;
@@ -1208,14 +1345,31 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         la.items = p->d_items;
         la.tab = c->d_tab;
         la.mixbits = p->d_mixbits;
-        la.nitems = p->nitems;
+        la.mon = p->d_mon;
+        la.p2m_dir = c->d_p2m_dir;
+        la.p2m_tgt = c->d_p2m_tgt;
+        la.p2m_part = c->d_p2m_part;
         la.inv_dx = 1.0 / p->dx;
         la.neg_inv_dx = -1.0 / p->dx;
-        if (p->type == OCTOFMM_RHO)
-            launch_level<true>(p->variant, p->nitems, st, la, p->precision == OCTOFMM_PRECISION_MIXED);
-        else
-            launch_level<false>(p->variant, p->nitems, st, la, p->precision == OCTOFMM_PRECISION_MIXED);
-        HIPC(c, hipGetLastError());
+        const bool rho = p->type == OCTOFMM_RHO;
+        if (p->n_m2l > 0) {
+            la.nitems = p->n_m2l;
+            la.item0 = 0;
+            if (rho)
+                launch_level<true>(p->variant, p->n_m2l, st, la, p->precision == OCTOFMM_PRECISION_MIXED);
+            else
+                launch_level<false>(p->variant, p->n_m2l, st, la, p->precision == OCTOFMM_PRECISION_MIXED);
+            HIPC(c, hipGetLastError());
+        }
+        if (p->nitems > p->n_m2l) {
+            la.nitems = p->nitems - p->n_m2l;
+            la.item0 = p->n_m2l;
+            if (rho)
+                launch_leaf<true>(la.nitems, st, la, c->tab.p2p_halo);
+            else
+                launch_leaf<false>(la.nitems, st, la, c->tab.p2p_halo);
+            HIPC(c, hipGetLastError());
+        }
     }
     return OCTOFMM_OK;
 }
@@ -1379,12 +1533,14 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
         st.p2m_pairs * (rho ? 318 : 205);
     std::vector<int2> items;
     for (int b = 0; b < 4; ++b) items.insert(items.end(), buckets[b].begin(), buckets[b].end());
+    const int32_t n_m2l = (int32_t)(buckets[0].size() + buckets[1].size());
 
     HIPC(c, hipSetDevice(c->device));
     octofmm_plan* p = new octofmm_plan();
     p->ctx = c;
     p->nslots = ns;
     p->nitems = (int32_t)items.size();
+    p->n_m2l = n_m2l;
     p->type = lv->type;
     p->precision = lv->precision;
     p->flags = lv->flags;

@@ -48,6 +48,13 @@ struct Tables {
     int32_t nnear_k[8];
     uint16_t farlist[8][kMaxDelta * 8];
     uint16_t nearlist[8][kMaxDelta * 8];
+    // p2p of a leaf target of class k: every stencil partner (far and near),
+    // packed as box_index | (window_offset + 2048) << 16, where the window is
+    // the sub-grid plus a halo of p2p_halo cells (edge 8 + 2 p2p_halo, x
+    // slowest) and window_offset = (dx * W + dy) * W + dz
+    int32_t p2p_halo;
+    int32_t np2p_k[8];
+    int32_t p2plist[8][kMaxDelta * 8];
 };
 
 // Level-kernel variants: lane mapping x 100 + waves per workgroup.

@@ -148,6 +148,22 @@ bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
         t.nfar_k[k] = a;
         t.nnear_k[k] = b;
     }
+    const int h = 2 * maxd + 1;
+    const int W = kInx + 2 * h;
+    t.p2p_halo = h;
+    for (int k = 0; k < 8; ++k) {
+        int q = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < 8; ++j) {
+                if (!(((t.farbits[i] | t.nearbits[i]) >> (8 * k + j)) & 1)) continue;
+                const int dx = 2 * t.delta[i][0] + ((j >> 2) & 1) - ((k >> 2) & 1);
+                const int dy = 2 * t.delta[i][1] + ((j >> 1) & 1) - ((k >> 1) & 1);
+                const int dz = 2 * t.delta[i][2] + (j & 1) - (k & 1);
+                const int woff = (dx * W + dy) * W + dz;
+                t.p2plist[k][q++] = box_index(dx, dy, dz) | ((woff + 2048) << 16);
+            }
+        t.np2p_k[k] = q;
+    }
     return true;
 }
 

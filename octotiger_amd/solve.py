@@ -1,0 +1,60 @@
+"""Run the whole-tree gravity solve on one GPU and print its timings as JSON.
+
+    python -m octotiger_amd.solve [--tree c5|small] [--steps N] [--precision fp64|mixed]
+
+Builds the tree on the host (octotiger_amd.workloads), uploads the leaf masses,
+creates the tree plan (one hipGraph per solve, autotuned level kernels) and
+replays it N times between HIP events; then times each phase (upward M2M,
+interactions, downward L2L + forces) separately.
+"""
+import argparse
+import json
+import sys
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--tree", choices=("c5", "small"), default="c5")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args(argv)
+
+    from . import _abi, hip, native, workloads
+
+    if hip.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        return 3
+    tree = workloads.config5() if args.tree == "c5" else workloads.small_tree()
+    ctx = native.Context(theta=0.5, device=0)
+    dev = hip.TreeOnDevice(tree)
+    flags = _abi.PLAN_USE_GRAPH | (0 if args.no_autotune else _abi.PLAN_AUTOTUNE)
+    prec = _abi.PRECISION_MIXED if args.precision == "mixed" else _abi.PRECISION_FP64
+    plan = dev.plan(ctx, precision=prec, flags=flags)
+    s = hip.Stream()
+    for _ in range(args.warmup):
+        plan.execute(s.handle)
+    s.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(s)
+    for _ in range(args.steps):
+        plan.execute(s.handle)
+    e1.record(s)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    phases = plan.time_phases(s.handle, iters=3)
+    st = plan.stats()
+    pairs = sum(st[k] for k in ("far_pairs", "near_pairs", "p2p_pairs", "p2m_pairs", "root_pairs"))
+    print(json.dumps({"tree": tree.name, "nodes_per_level": tree.info["nodes_per_level"],
+                      "precision": args.precision, "ms_per_solve": ms,
+                      "phases_ms": dict(zip(("upward_m2m", "interactions", "downward_l2l_force"), phases)),
+                      "cell_interactions_per_s": pairs / (ms * 1e-3), "stats": st}), flush=True)
+    plan.close()
+    dev.free()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -127,6 +127,22 @@ def test_level_mixed_kinds(ctx, t):
         assert normwise(Lc[tg], Lcr[tg]) <= TOL
 
 
+@pytest.mark.parametrize("theta", [0.34, 0.45])
+def test_level_mixed_kinds_other_theta(theta):
+    """Wider stencils (theta 0.34: coarse offsets up to 2, p2p window 18^3)."""
+    c = native.Context(theta=theta, device=0)
+    lv = workloads.mixed_level((5, 5, 5), seed=13, leaf_fraction=0.5)
+    print(f"mixed level theta {theta} {lv.info}", flush=True)
+    outs, stats = run_level(c, lv, _abi.RHO)
+    c.close()
+    L, Lc = outs[0]
+    Lr, Lcr = oracle.level_interactions(lv, theta=theta, type_=_abi.RHO)
+    tg = lv.targets
+    assert stats["p2m_pairs"] > 0 and stats["p2p_pairs"] > 0 and stats["near_pairs"] > 0
+    assert normwise(L[tg], Lr[tg]) <= TOL
+    assert normwise(Lc[tg], Lcr[tg]) <= TOL
+
+
 def test_level_graph_replays_identical(ctx):
     lv = workloads.mixed_level((5, 5, 5), seed=5, leaf_fraction=0.4)
     outs, _ = run_level(ctx, lv, _abi.RHO, repeats=3)
