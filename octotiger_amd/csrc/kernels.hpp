@@ -37,6 +37,8 @@ __host__ __device__ constexpr double record_scale(int c) {
 struct DTensor {
     double D[20];
     double d2, d3;
+    // d3 times monomials of the separation, shared with the correction
+    double d3x00, d3x11, d3x22, d3x01, d3x02, d3x12;
 };
 
 // 1/sqrt(x) to full double precision: hardware estimate plus two Newton steps.
@@ -74,12 +76,20 @@ __device__ __forceinline__ void d_tensor(double x, double y, double z, DTensor& 
     D[8] = d2 * (y * z);
     D[9] = fma(d2, x22, d1);
     const double d2x = d2 * x, d2y = d2 * y, d2z = d2 * z;
-    const double d3x00 = d3 * x00, d3x11 = d3 * x11, d3x22 = d3 * x22;
+    const double d3x = d3 * x, d3y = d3 * y;
+    const double d3x00 = d3x * x, d3x11 = d3y * y, d3x22 = (d3 * z) * z;
+    const double d3x01 = d3x * y, d3x02 = d3x * z, d3x12 = d3y * z;
+    t.d3x00 = d3x00;
+    t.d3x11 = d3x11;
+    t.d3x22 = d3x22;
+    t.d3x01 = d3x01;
+    t.d3x02 = d3x02;
+    t.d3x12 = d3x12;
     D[10] = fma(d3x00, x, 3.0 * d2x);
     D[11] = fma(d3x00, y, d2y);
     D[12] = fma(d3x00, z, d2z);
     D[13] = fma(d3x11, x, d2x);
-    D[14] = (d3 * x) * (y * z);
+    D[14] = d3x12 * x;
     D[15] = fma(d3x22, x, d2x);
     D[16] = fma(d3x11, y, 3.0 * d2y);
     D[17] = fma(d3x11, z, d2z);
@@ -130,11 +140,8 @@ __device__ __forceinline__ void dipole_rows(const double* m, const double* D, do
 // Angular-momentum correction from the order-4 tensor (compute_interaction_multipole_rho,
 // :224-334; compute_interaction_p2m_rho, monopole_kernel_templates.hpp:69-187).
 // a[k] = factor-scaled octupole difference (M2L) or partner octupole (p2m).
-__device__ __forceinline__ void am_correction(double x, double y, double z, double d2, double d3,
-    const double* a, double* C) {
-    const double d3x = d3 * x, d3y = d3 * y;
-    const double d3x00 = d3x * x, d3x11 = d3y * y, d3x22 = (d3 * z) * z;
-    const double d3x01 = d3x * y, d3x02 = d3x * z, d3x12 = d3y * z;
+__device__ __forceinline__ void am_correction_t(double d2, double d3x00, double d3x11, double d3x22, double d3x01,
+    double d3x02, double d3x12, const double* a, double* C) {
     const double t2 = 3.0 * d2;
     const double U0 = fma(6.0, d3x00, t2);
     const double U1 = 3.0 * d3x01;
@@ -160,6 +167,14 @@ __device__ __forceinline__ void am_correction(double x, double y, double z, doub
     c0 = fma(-a[8], U8, c0);  c1 = fma(-a[8], U12, c1); c2 = fma(-a[8], U13, c2);
     c0 = fma(-a[9], U2, c0);  c1 = fma(-a[9], U13, c1); c2 = fma(-a[9], U14, c2);
     C[0] = c0; C[1] = c1; C[2] = c2;
+}
+
+// The same correction from the separation and d2, d3 (used where no
+// derivative tensor with the shared products is at hand).
+__device__ __forceinline__ void am_correction(double x, double y, double z, double d2, double d3,
+    const double* a, double* C) {
+    const double d3x = d3 * x, d3y = d3 * y;
+    am_correction_t(d2, d3x * x, d3y * y, (d3 * z) * z, d3x * y, d3x * z, d3y * z, a, C);
 }
 
 __device__ __forceinline__ void load_record20(const double* __restrict__ p, double* m) {
@@ -206,7 +221,7 @@ __device__ __forceinline__ void m2l_far_rec(const Rec& r, double X0, double X1, 
         double a[10];
 #pragma unroll
         for (int k = 0; k < 10; ++k) a[k] = fma(-mc[k], n0c, m[10 + k]);
-        am_correction(x, y, z, t.d2, t.d3, a, C);
+        am_correction_t(t.d2, t.d3x00, t.d3x11, t.d3x22, t.d3x01, t.d3x02, t.d3x12, a, C);
     }
 }
 
@@ -330,7 +345,7 @@ __device__ __forceinline__ void m2l_far(const double* __restrict__ p, double X0,
         double a[10];
 #pragma unroll
         for (int k = 0; k < 10; ++k) a[k] = fma(-mc[k], n0c, m[10 + k]);
-        am_correction(x, y, z, t.d2, t.d3, a, C);
+        am_correction_t(t.d2, t.d3x00, t.d3x11, t.d3x22, t.d3x01, t.d3x02, t.d3x12, a, C);
     }
 }
 
@@ -348,7 +363,7 @@ __device__ __forceinline__ void m2l_near(double mn, double Y0, double Y1, double
         double a[10];
 #pragma unroll
         for (int k = 0; k < 10; ++k) a[k] = -(mc[k] * n0c);
-        am_correction(x, y, z, t.d2, t.d3, a, C);
+        am_correction_t(t.d2, t.d3x00, t.d3x11, t.d3x22, t.d3x01, t.d3x02, t.d3x12, a, C);
     }
 }
 
@@ -365,7 +380,7 @@ __device__ __forceinline__ void p2m_pair(const double* __restrict__ p, double X0
     for (int i = 0; i < 4; ++i) L[i] = fma(m[0], t.D[i], L[i]);
     if constexpr (!RHO) dipole_rows(m, t.D, L);
     contract_high<4>(m, t.D, L);
-    if constexpr (RHO) am_correction(x, y, z, t.d2, t.d3, m + 10, C);
+    if constexpr (RHO) am_correction_t(t.d2, t.d3x00, t.d3x11, t.d3x22, t.d3x01, t.d3x02, t.d3x12, m + 10, C);
 }
 
 // ---------------------------------------------------------------------------
@@ -379,8 +394,9 @@ struct StageArgs {
     const double* fxbase;   // [nslots][3] frame origin for leaf-cell centres
     const int32_t* foff;    // [nslots][3] frame offset of the slot's cell (0,0,0)
     double* rec;            // [nslots][512][24]
+    const int32_t* list;    // slots to stage (null: all nslots)
     double dx;
-    int32_t nslots;
+    int32_t nslots;         // slots to stage (length of list when given)
 };
 
 struct LevelArgs {

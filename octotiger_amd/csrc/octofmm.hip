@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void stage_records(StageArgs a) {
 #pragma clang fp contract(off)
     const int gid = blockIdx.x * 256 + threadIdx.x;
     if (gid >= a.nslots * kCells) return;
-    const int slot = gid >> 9, cell = gid & 511;
+    const int slot = a.list ? a.list[gid >> 9] : gid >> 9, cell = gid & 511;
     const int x = cell >> 6, y = (cell >> 3) & 7, z = cell & 7;
     const int r = (((x >> 1) << 4) | ((y >> 1) << 2) | (z >> 1)) * 8 + (((x & 1) << 2) | ((y & 1) << 1) | (z & 1));
     double* o = a.rec + ((size_t)slot * kCells + r) * kRecDoubles;
@@ -348,6 +348,254 @@ __device__ void m2l_item_lds(const LevelArgs& a, int slot, int flags, const int*
     }
 }
 
+// Mapping 3: mapping 2 with the staging done by direct global->LDS loads
+// (global_load_lds_dwordx4): the 64 partner records a wave needs for one
+// coarse offset (8 partner coarse cells x 8 children, 12 KiB) go from L2
+// into the wave's LDS image without passing through registers, 16 bytes per
+// lane and instruction; the 8 lanes of a coarse cell then read each record as
+// an LDS broadcast.  Each group of 96 chunks is followed by one unused pad
+// chunk so the 8 groups read together fall into different banks.
+constexpr int kDirGroup = 97;              // 96 chunks + 1 pad
+constexpr int kDirWave = 8 * kDirGroup;    // 776 double2 = 12.1 KiB per wave
+
+typedef __attribute__((address_space(3))) void* lds_ptr;
+// Read-only data seen through the constant address space: uniform loads stay
+// scalar even after the kernel has issued LDS-direct loads.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
+
+template <bool RHO>
+__device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind,
+    int w, double2* __restrict__ s_stage) {
+    // this wave's image (wave-uniform address)
+    double2* stage = s_stage + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * kDirWave;
+    const LaneGeom g = lane_geom(w);
+    const int l = threadIdx.x & 63, c = l >> 3;
+    const auto* tab = as_const(a.tab);
+    const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
+    const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
+    const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
+    double mc[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
+    double L[20], C[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 20; ++i) L[i] = 0.0;
+    const int nd = tab->ndelta;
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const int bx = ((wu >> 2) & 1) * 2, by = ((wu >> 1) & 1) * 2, bz = (wu & 1) * 2;
+    const auto* nbr_row = as_const(a.nbr + (size_t)__builtin_amdgcn_readfirstlane(slot) * 27);
+    auto group_base = [&](int gi, int ax, int ay, int az) -> const double2* {
+        const int px = bx + ((gi >> 2) & 1) + ax, py = by + ((gi >> 1) & 1) + ay, pz = bz + (gi & 1) + az;
+        const int raw = (px < 0 ? 0 : (px < 4 ? 1 : 2)) + 3 * (py < 0 ? 0 : (py < 4 ? 1 : 2)) +
+            9 * (pz < 0 ? 0 : (pz < 4 ? 1 : 2));
+        const int ns = nbr_row[raw];
+        const int cb = ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
+        return reinterpret_cast<const double2*>(a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles);
+    };
+    // Group c (8 records = 96 chunks) is loaded by one full instruction
+    // (chunks 0..63) and one on the lower 32 lanes (chunks 64..95): both are a
+    // uniform base plus the lane's 16-byte offset, and the LDS destination is
+    // uniform too, so a refill costs no per-lane address arithmetic.
+    const unsigned loff = (unsigned)l * 16u;
+    auto issue = [&](int di) {
+        const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+#pragma unroll
+        for (int gi = 0; gi < 8; ++gi) {
+            const char* b = reinterpret_cast<const char*>(group_base(gi, ax, ay, az));
+            __builtin_amdgcn_global_load_lds(b + loff, (lds_ptr)(stage + gi * kDirGroup), 16, 0, 0);
+            if (l < 32)
+                __builtin_amdgcn_global_load_lds(b + 1024 + loff, (lds_ptr)(stage + gi * kDirGroup + 64), 16, 0, 0);
+        }
+    };
+    auto far_bits = [&](int di) -> unsigned {
+        unsigned fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
+        int ns, raw;
+        partner_base(g, tab->delta[di][0], tab->delta[di][1], tab->delta[di][2], s_nbr, ns, raw);
+        return ns < 0 ? 0u : fb;
+    };
+    const double2* group = stage + c * kDirGroup;
+    issue(0);
+    for (int di = 0; di < nd; ++di) {
+        const unsigned fb = far_bits(di);
+        __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): this wave's image is in LDS
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (fb & (1u << j)) {
+                Rec cur;
+#pragma unroll
+                for (int i = 0; i < 12; ++i) cur.v[i] = group[j * 12 + i];
+                m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
+            }
+        }
+        lds_wave_sync();   // every read of the image done before it is refilled
+        if (di + 1 < nd) issue(di + 1);
+    }
+    if (flags & kItemNear) {
+        for (int di = 0; di < nd; ++di) {
+            const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+            unsigned nb = (unsigned)(tab->nearbits[di] >> (8 * g.k)) & 0xffu;
+            int ns, raw;
+            const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
+            if (ns < 0 || s_kind[raw] != OCTOFMM_SLOT_LEAF) nb = 0;
+            const double* pb2 = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
+#pragma unroll 1
+            for (int j = 0; j < 8; ++j) {
+                if (nb & (1u << j)) {
+                    const double* p = pb2 + j * kRecDoubles;
+                    m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, inv_mc0, mc, L, C);
+                }
+            }
+        }
+    }
+    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
+#pragma unroll
+    for (int i = 0; i < 20; ++i) Lo[i] += L[i];
+    if (RHO) {
+        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
+        Co[0] = C[0];
+        Co[1] = C[1];
+        Co[2] = C[2];
+    }
+}
+
+// Mapping 4: mapping 3 with the image split by child halves and double
+// buffered.  Buffer h holds children 4h..4h+3 of the 8 partner coarse cells
+// (8 groups of 48 chunks + 1 pad chunk); while the wave computes with one
+// half, the other half is being refilled, so a refill's latency overlaps
+// four pairs of arithmetic instead of stalling the wave.
+constexpr int kHalfGroup = 49;                       // 48 chunks + 1 pad
+constexpr int kHalfWave = 8 * kHalfGroup;            // 392 double2
+
+// One direct global->LDS load of 16 bytes per active lane: lane i writes LDS
+// bytes [lds + 16 i, lds + 16 i + 16) from gbase + voff.  Issued as inline
+// assembly so the compiler does not track it: the caller orders it with
+// explicit waits (vmcnt before reading the bytes, LDS reads drained before
+// refilling them).
+__device__ __forceinline__ void lds_dma16(const void* gbase, unsigned voff, unsigned lds) {
+    asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1"
+                 :
+                 : "v"(voff), "s"(gbase), "s"(lds)
+                 : "memory", "m0");
+}
+
+__device__ __forceinline__ unsigned lds_offset(const void* p) {
+    return (unsigned)(size_t)(lds_ptr)p;
+}
+
+template <bool RHO>
+__device__ void m2l_item_direct2(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind,
+    int w, double2* __restrict__ s_half0, double2* __restrict__ s_half1) {
+    // the two halves live in two distinct LDS arrays, so reads of one half
+    // need not wait for the refill of the other
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    double2* const half[2] = {s_half0 + wv * kHalfWave, s_half1 + wv * kHalfWave};
+    const LaneGeom g = lane_geom(w);
+    const int l = threadIdx.x & 63, c = l >> 3;
+    const auto* tab = as_const(a.tab);
+    const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
+    const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
+    const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
+    double mc[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
+    double L[20], C[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 20; ++i) L[i] = 0.0;
+    const int nd = tab->ndelta;
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const int bx = ((wu >> 2) & 1) * 2, by = ((wu >> 1) & 1) * 2, bz = (wu & 1) * 2;
+    const auto* nbr_row = as_const(a.nbr + (size_t)__builtin_amdgcn_readfirstlane(slot) * 27);
+    auto group_base = [&](int gi, int ax, int ay, int az) -> const char* {
+        const int px = bx + ((gi >> 2) & 1) + ax, py = by + ((gi >> 1) & 1) + ay, pz = bz + (gi & 1) + az;
+        const int raw = (px < 0 ? 0 : (px < 4 ? 1 : 2)) + 3 * (py < 0 ? 0 : (py < 4 ? 1 : 2)) +
+            9 * (pz < 0 ? 0 : (pz < 4 ? 1 : 2));
+        const int ns = nbr_row[raw];
+        const int cb = ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
+        return reinterpret_cast<const char*>(a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles);
+    };
+    // 8 loads on the lower 48 lanes: chunk l of children 4h..4h+3 of group gi
+    const unsigned loff = (unsigned)l * 16u;
+    auto issue = [&](int di, int h) {
+        const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+        double2* buf = half[h];
+#pragma unroll
+        for (int gi = 0; gi < 8; ++gi) {
+            const char* b = group_base(gi, ax, ay, az) + h * 48 * 16;
+            const unsigned dst = (unsigned)__builtin_amdgcn_readfirstlane((int)lds_offset(buf + gi * kHalfGroup));
+            if (l < 48) lds_dma16(b, loff, dst);
+        }
+    };
+    auto far_bits = [&](int di) -> unsigned {
+        unsigned fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
+        int ns, raw;
+        partner_base(g, tab->delta[di][0], tab->delta[di][1], tab->delta[di][2], s_nbr, ns, raw);
+        return ns < 0 ? 0u : fb;
+    };
+    // the target's own loads complete before the first refill, so no later
+    // wait for them can drain the refills in flight
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0)
+    asm volatile("" ::: "memory");
+    issue(0, 0);
+    issue(0, 1);
+    for (int di = 0; di < nd; ++di) {
+        const unsigned fb = far_bits(di);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            // this half's 8 loads are done; the other half's 8 may still fly
+            __builtin_amdgcn_s_waitcnt(0x3F78);   // vmcnt(8)
+            __builtin_amdgcn_wave_barrier();
+            const double2* group = half[h] + c * kHalfGroup;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (fb & (1u << (4 * h + r))) {
+                    Rec cur;
+#pragma unroll
+                    for (int i = 0; i < 12; ++i) cur.v[i] = group[r * 12 + i];
+                    m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
+                }
+            }
+            lds_wave_sync();   // every read of this half done before it is refilled
+            if (di + 1 < nd) {
+                issue(di + 1, h);
+            } else {
+                __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): nothing more in flight to wait behind
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+    if (flags & kItemNear) {
+        for (int di = 0; di < nd; ++di) {
+            const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+            unsigned nb = (unsigned)(tab->nearbits[di] >> (8 * g.k)) & 0xffu;
+            int ns, raw;
+            const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
+            if (ns < 0 || s_kind[raw] != OCTOFMM_SLOT_LEAF) nb = 0;
+            const double* pb2 = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
+#pragma unroll 1
+            for (int j = 0; j < 8; ++j) {
+                if (nb & (1u << j)) {
+                    const double* p = pb2 + j * kRecDoubles;
+                    m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, inv_mc0, mc, L, C);
+                }
+            }
+        }
+    }
+    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
+#pragma unroll
+    for (int i = 0; i < 20; ++i) Lo[i] += L[i];
+    if (RHO) {
+        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
+        Co[0] = C[0];
+        Co[1] = C[1];
+        Co[2] = C[2];
+    }
+}
+
 // Mapping 1: wave = child class k over the 64 coarse cells of the sub-grid;
 // the (delta, j) partner list is the same for every lane.
 template <bool RHO>
@@ -488,9 +736,11 @@ __global__ __launch_bounds__(512) void leaf_kernel(LevelArgs a) {
     const bool p2m = (it.y & kItemP2M) != 0;
     if (p2m) {
         // refined neighbours, one direction at a time: each listed target of
-        // the direction is one thread, its sums go to the target's LDS slot;
-        // a barrier separates directions, so every target's sum is formed in
-        // the same order on every run
+        // the direction gets 8 lanes, which stride over its partners and
+        // combine their sums by a fixed butterfly; lane 0 of the group adds
+        // the result to the target's LDS slot.  A barrier separates
+        // directions, so every target's sum is formed in the same order on
+        // every run.
         double* acc = s_win;   // the p2p window is no longer needed
         __syncthreads();
         for (int i = threadIdx.x; i < kCells * 7; i += 512) acc[i] = 0.0;
@@ -499,22 +749,24 @@ __global__ __launch_bounds__(512) void leaf_kernel(LevelArgs a) {
             const int ns = s_nbr[r];
             if (r == 13 || ns < 0 || s_kind[r] != OCTOFMM_SLOT_NONLEAF) continue;
             const int nt = a.p2m_dir[2 * r], t0 = a.p2m_dir[2 * r + 1];
-            for (int q = threadIdx.x; q < nt; q += 512) {
-                const int4 te = a.p2m_tgt[t0 + q];
+            for (int q = threadIdx.x; q < nt * 8; q += 512) {
+                const int sub = q & 7;
+                const int4 te = a.p2m_tgt[t0 + (q >> 3)];
                 const double* xc = a.com + ((size_t)slot * kCells + te.x) * 3;
                 const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
-                double Lp[4] = {0.0, 0.0, 0.0, 0.0}, Cp[3] = {0.0, 0.0, 0.0};
+                double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
                 const double* base = a.rec + (size_t)ns * kCells * kRecDoubles;
-                for (int e = 0; e < te.z; ++e)
-                    p2m_pair<RHO>(base + (size_t)a.p2m_part[te.y + e] * kRecDoubles, X0, X1, X2, Lp, Cp);
-                double* o = acc + te.x * 7;
-                o[0] += Lp[0];
-                o[1] += Lp[1];
-                o[2] += Lp[2];
-                o[3] += Lp[3];
-                o[4] += Cp[0];
-                o[5] += Cp[1];
-                o[6] += Cp[2];
+                for (int e = sub; e < te.z; e += 8)
+                    p2m_pair<RHO>(base + (size_t)a.p2m_part[te.y + e] * kRecDoubles, X0, X1, X2, v, v + 4);
+#pragma unroll
+                for (int m = 1; m < 8; m <<= 1)
+#pragma unroll
+                    for (int i = 0; i < 7; ++i) v[i] += __shfl_xor(v[i], m, 8);
+                if (sub == 0) {
+                    double* o = acc + te.x * 7;
+#pragma unroll
+                    for (int i = 0; i < 7; ++i) o[i] += v[i];
+                }
             }
             __syncthreads();
         }
@@ -574,6 +826,31 @@ __global__ __launch_bounds__(64 * WAVES) void level_kernel(LevelArgs a) {
         m2l_item<RHO, MIXED>(a, it.x, it.y, s_nbr, s_kind, w);
 }
 
+// Mapping-3/4 kernel (direct global->LDS staging); one wave per 2x2x2 coarse block.
+template <bool RHO, int WAVES, int MAPPING>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3))) void level_kernel_direct(LevelArgs a) {
+    constexpr int kParts = 8 / WAVES;
+    __shared__ int s_nbr[27];
+    __shared__ int s_kind[27];
+    __shared__ double2 s_stage[MAPPING == 4 ? WAVES * kHalfWave : WAVES * kDirWave];
+    __shared__ double2 s_stage1[MAPPING == 4 ? WAVES * kHalfWave : 1];
+    const int v = xcd_item(blockIdx.x, gridDim.x);
+    const int item = v / kParts;
+    const int w = (v % kParts) * WAVES + (threadIdx.x >> 6);
+    const int2 it = a.items[a.item0 + item];
+    if (threadIdx.x < 27) {
+        const int n = a.nbr[(size_t)it.x * 27 + threadIdx.x];
+        s_nbr[threadIdx.x] = n;
+        s_kind[threadIdx.x] = n >= 0 ? a.kind[n] : -1;
+    }
+    __syncthreads();
+    if (it.y & kItemLeaf) return;
+    if (MAPPING == 4)
+        m2l_item_direct2<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage, s_stage1);
+    else
+        m2l_item_direct<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage);
+}
+
 template <bool RHO>
 void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, bool mixed = false) {
     const int waves = variant % 100;
@@ -594,6 +871,10 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
     case 108: hipLaunchKernelGGL((level_kernel<RHO, 8, 1>), grid, block, 0, st, la); break;
     case 104: hipLaunchKernelGGL((level_kernel<RHO, 4, 1>), grid, block, 0, st, la); break;
     case 102: hipLaunchKernelGGL((level_kernel<RHO, 2, 1>), grid, block, 0, st, la); break;
+    case 304: hipLaunchKernelGGL((level_kernel_direct<RHO, 4, 3>), grid, block, 0, st, la); break;
+    case 302: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 3>), grid, block, 0, st, la); break;
+    case 404: hipLaunchKernelGGL((level_kernel_direct<RHO, 4, 4>), grid, block, 0, st, la); break;
+    case 402: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 4>), grid, block, 0, st, la); break;
     case 208: hipLaunchKernelGGL((level_kernel<RHO, 8, 2>), grid, block, 0, st, la); break;
     case 204: hipLaunchKernelGGL((level_kernel<RHO, 4, 2>), grid, block, 0, st, la); break;
     default: hipLaunchKernelGGL((level_kernel<RHO, 2, 2>), grid, block, 0, st, la); break;
@@ -831,6 +1112,8 @@ struct octofmm_plan {
     octofmm_context* ctx = nullptr;
     int32_t nslots = 0, nitems = 0, type = 0, precision = 0, flags = 0;
     int32_t n_m2l = 0;             // items [0, n_m2l) are non-leaf, the rest leaf targets
+    int32_t n_stage = 0;           // slots whose partner records some item reads
+    int32_t* d_stage_list = nullptr;   // those slots, or null when all are read
     double dx = 0.0;
     const double* d_M = nullptr;
     const double* d_com = nullptr;
@@ -855,6 +1138,9 @@ struct octofmm_plan {
 namespace {
 
 constexpr int kRootChunks = 64;
+// record buffers end with a tail the direct-to-LDS staging may read into
+// (one 16-byte pad chunk past the last record)
+constexpr size_t kRecTailBytes = 4096;
 // layout of the per-sub-grid workspace metadata (int32 words); the int2 work
 // item comes first so it stays 8-byte aligned
 constexpr int kMetaItems = 0;
@@ -896,7 +1182,7 @@ octofmm_status ensure_workspace(octofmm_context* c) {
     HIPC(c, hipMalloc((void**)&c->d_M, sizeof(double) * 27 * kCells * 20));
     HIPC(c, hipMalloc((void**)&c->d_com, sizeof(double) * 27 * kCells * 3));
     HIPC(c, hipMalloc((void**)&c->d_mon, sizeof(double) * 27 * kCells));
-    HIPC(c, hipMalloc((void**)&c->d_rec, sizeof(double) * 27 * kCells * kRecDoubles));
+    HIPC(c, hipMalloc((void**)&c->d_rec, sizeof(double) * 27 * kCells * kRecDoubles + kRecTailBytes));
     HIPC(c, hipMalloc((void**)&c->d_L, sizeof(double) * 27 * kCells * 20));
     HIPC(c, hipMalloc((void**)&c->d_Lc, sizeof(double) * 27 * kCells * 3));
     HIPC(c, hipMalloc((void**)&c->d_meta, sizeof(int32_t) * kMetaSize));
@@ -1065,6 +1351,7 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
     sa.fxbase = c->d_fx;
     sa.foff = c->d_meta + kMetaFoff;
     sa.rec = c->d_rec;
+    sa.list = nullptr;
     sa.dx = dx;
     sa.nslots = 27;
     hipLaunchKernelGGL(stage_records, dim3(27 * kCells / 256), dim3(256), 0, st, sa);
@@ -1294,6 +1581,7 @@ static void plan_free(octofmm_plan* p) {
     (void)hipFree(p->d_items);
     (void)hipFree(p->d_mixbits);
     (void)hipFree(p->d_partial);
+    (void)hipFree(p->d_stage_list);
     delete p;
 }
 
@@ -1308,11 +1596,14 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         sa.fxbase = p->d_fxbase;
         sa.foff = p->d_foff;
         sa.rec = p->d_rec;
+        sa.list = p->d_stage_list;
         sa.dx = p->dx;
-        sa.nslots = p->nslots;
-        const int nthreads = p->nslots * kCells;
-        hipLaunchKernelGGL(stage_records, dim3((nthreads + 255) / 256), dim3(256), 0, st, sa);
-        HIPC(c, hipGetLastError());
+        sa.nslots = p->d_stage_list ? p->n_stage : p->nslots;
+        const int nthreads = sa.nslots * kCells;
+        if (nthreads > 0) {
+            hipLaunchKernelGGL(stage_records, dim3((nthreads + 255) / 256), dim3(256), 0, st, sa);
+            HIPC(c, hipGetLastError());
+        }
     }
     if ((stage_mask & 2) && p->root) {
         RootArgs ra;
@@ -1534,6 +1825,26 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
     std::vector<int2> items;
     for (int b = 0; b < 4; ++b) items.insert(items.end(), buckets[b].begin(), buckets[b].end());
     const int32_t n_m2l = (int32_t)(buckets[0].size() + buckets[1].size());
+    // partner records are read by non-leaf targets (their own slot and every
+    // neighbour) and by leaf targets with refined neighbours (those
+    // neighbours); pure p2p leaves read monopoles directly
+    std::vector<int32_t> stage_list;
+    {
+        std::vector<char> need(ns, 0);
+        for (const int2& it : items) {
+            const int32_t* nb = lv->neighbors + (size_t)it.x * 27;
+            if (!(it.y & kItemLeaf)) {
+                for (int r = 0; r < 27; ++r)
+                    if (nb[r] >= 0) need[nb[r]] = 1;
+            } else if (it.y & kItemP2M) {
+                for (int r = 0; r < 27; ++r)
+                    if (nb[r] >= 0 && lv->kind[nb[r]] == OCTOFMM_SLOT_NONLEAF) need[nb[r]] = 1;
+            }
+        }
+        if (root) need[0] = 1;
+        for (int s2 = 0; s2 < ns; ++s2)
+            if (need[s2]) stage_list.push_back(s2);
+    }
 
     HIPC(c, hipSetDevice(c->device));
     octofmm_plan* p = new octofmm_plan();
@@ -1541,6 +1852,7 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
     p->nslots = ns;
     p->nitems = (int32_t)items.size();
     p->n_m2l = n_m2l;
+    p->n_stage = (int32_t)stage_list.size();
     p->type = lv->type;
     p->precision = lv->precision;
     p->flags = lv->flags;
@@ -1558,7 +1870,7 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
         plan_free(p);
         return fail(c, s, "%s", what);
     };
-    if (hipMalloc((void**)&p->d_rec, (size_t)ns * kCells * kRecDoubles * sizeof(double)) != hipSuccess)
+    if (hipMalloc((void**)&p->d_rec, (size_t)ns * kCells * kRecDoubles * sizeof(double) + kRecTailBytes) != hipSuccess)
         return cleanup(OCTOFMM_ERR_ALLOCATION, "record buffer allocation failed");
     if (hipMalloc((void**)&p->d_nbr, sizeof(int32_t) * 27 * ns) != hipSuccess ||
         hipMalloc((void**)&p->d_kind, sizeof(int32_t) * ns) != hipSuccess ||
@@ -1575,6 +1887,12 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
         return cleanup(OCTOFMM_ERR_HIP, "plan table upload failed");
     if (root && hipMalloc((void**)&p->d_partial, sizeof(double) * kRootChunks * kCells * kRecDoubles) != hipSuccess)
         return cleanup(OCTOFMM_ERR_ALLOCATION, "root partial buffer allocation failed");
+    if ((int)stage_list.size() < ns) {
+        if (hipMalloc((void**)&p->d_stage_list, sizeof(int32_t) * std::max<size_t>(1, stage_list.size())) != hipSuccess ||
+            (!stage_list.empty() && hipMemcpy(p->d_stage_list, stage_list.data(), sizeof(int32_t) * stage_list.size(),
+                                        hipMemcpyHostToDevice) != hipSuccess))
+            return cleanup(OCTOFMM_ERR_ALLOCATION, "staging list upload failed");
+    }
     if (p->precision == OCTOFMM_PRECISION_MIXED) {
         // far pairs with |d|^2 > mixed_far_d2 (cell units) are evaluated in FP32
         std::vector<uint64_t> mb(T.ndelta > 0 ? T.ndelta : 1, 0);

@@ -64,7 +64,11 @@ struct Tables {
 //              partner list is uniform, so no lane idles on near pairs.
 //   mapping 2: mapping 0 with each coarse offset's partner records staged
 //              once per wave in LDS and read back as broadcasts.
-constexpr int kVariants[9] = {8, 4, 2, 108, 104, 102, 208, 204, 202};
+//   mapping 3: mapping 2 with the staging done by direct global->LDS loads
+//              (no staging registers); 4 or 2 waves per workgroup so that 3
+//              waves per SIMD fit the 12 KiB LDS image of each wave.
+//   mapping 4: mapping 3 with the image double buffered by child halves.
+constexpr int kVariants[13] = {8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402};
 
 // Builds the tables; returns false when theta is below the floor or the
 // stencil fails the reference's symmetry / width checks.
