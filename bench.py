@@ -47,6 +47,15 @@ def parse():
     return ap.parse_args()
 
 
+def kernel_name(variant):
+    """Device kernel the plan's interaction launch runs for a variant
+    (lane mapping x 100 + waves per workgroup)."""
+    mapping, waves = divmod(variant, 100)
+    if mapping >= 3:
+        return f"level_kernel_direct<RHO, {waves}, {mapping}>"
+    return f"level_kernel<RHO, {waves}, {mapping}>"
+
+
 def time_plan(plan, stream, steps, warmup, hip):
     for _ in range(warmup):
         plan.execute(stream.handle)
@@ -139,8 +148,8 @@ def run_tree(ctx, args, hip, _abi, np):
         stream.synchronize()
         g = np.concatenate([d["g"].numpy()[lv.kind == _abi.SLOT_LEAF].reshape(-1, 4)
                             for lv, d in zip(t.levels, dev.levels)])
-        res[name] = {"ms_per_solve": ms, "phases_ms": {"upward_m2m": phases[0], "interactions": phases[1],
-                                                       "downward_l2l_force": phases[2]},
+        res[name] = {"ms_per_solve": ms, "phases_ms": {"upward_m2m": phases[0],
+                                                       "interactions_l2l_forces": phases[1]},
                      "stats": plan.stats(), "g": g}
         plan.close()
     dev.free()
@@ -149,7 +158,7 @@ def run_tree(ctx, args, hip, _abi, np):
     ga, gb = res["fp64"]["g"], res["mixed"]["g"]
     ef = float(np.sqrt(((gb[:, 1:] - ga[:, 1:]) ** 2).sum() / (ga[:, 1:] ** 2).sum()))
     ephi = float(np.abs(gb[:, 0] - ga[:, 0]).max() / np.abs(ga[:, 0]).max())
-    inter = res["fp64"]["phases_ms"]["interactions"]
+    inter = res["fp64"]["phases_ms"]["interactions_l2l_forces"]
     return {
         "workload": f"{t.name}: n=1.5 polytrope (radius 0.3, 1% log-normal noise), {t.nnodes} sub-grids, "
                     f"nodes per level {t.info['nodes_per_level']}, leaves per level {t.info['leaves_per_level']}",
@@ -203,7 +212,9 @@ def main():
     achieved = st["flops"] / (k_ms * 1e-3) / 1e12
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{lv.name}.json")
+    # HBM bytes per launch of this kernel variant, from the calibrated
+    # FETCH_SIZE / WRITE_SIZE counter runs kept beside the profiles
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{lv.name}_v{int(st['variant'])}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -301,7 +312,7 @@ def main():
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "level_kernel<RHO>",
+            "kernel": kernel_name(int(st["variant"])),
             "achieved": achieved,
             "peak": FP64_VALU_PEAK_TFLOPS,
             "unit": "TFLOP/s",

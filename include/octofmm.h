@@ -97,8 +97,14 @@ typedef struct octofmm_subgrid {
 typedef struct octofmm_config {
     double theta;          /* opening criterion, Octo-Tiger default 0.5 */
     int32_t device;        /* HIP device ordinal */
-    int32_t reserved;
+    int32_t flags;         /* OCTOFMM_CHECK_FINITE or 0 */
 } octofmm_config;
+
+/* The per-sub-grid entry points reject non-finite inputs (M, com, mon of the
+ * sub-grid and of every neighbour handed over) with OCTOFMM_ERR_NONFINITE
+ * before any device work.  No counterpart in the reference, which does not
+ * check; off by default (one host pass over ~2.6 MB per call). */
+#define OCTOFMM_CHECK_FINITE 1
 
 typedef struct octofmm_context octofmm_context;
 
@@ -220,10 +226,10 @@ octofmm_status octofmm_level_plan_destroy(octofmm_plan* plan);
  * one hipGraph:
  *   1. upward M2M, finest level first (grid::compute_multipoles,
  *      src/grid_fmm.cpp:1639-1803, taylor::operator>>=, octotiger/taylor.hpp:222-255);
- *   2. interactions of every level (level plans; the root sub-grid with the
- *      root stencil);
- *   3. downward L2L, root first, with the leaf force assembly
- *      (grid::compute_expansions, src/grid_fmm.cpp:1524-1637,
+ *   2. interactions of every level, root first (level plans; the root
+ *      sub-grid with the root stencil), each level's kernels also applying
+ *      the downward L2L from the finished level above and the leaf force
+ *      assembly (grid::compute_expansions, src/grid_fmm.cpp:1524-1637,
  *      taylor::operator<<=, octotiger/taylor.hpp:263-311).
  * Level 0 holds the root alone.  Every non-leaf node of level l has exactly 8
  * children on level l+1; the neighbours of a non-leaf node exist on its level
@@ -284,7 +290,8 @@ octofmm_status octofmm_tree_plan_create(octofmm_context* ctx, const octofmm_tree
 octofmm_status octofmm_tree_plan_execute(octofmm_tree_plan* plan, void* stream);
 octofmm_status octofmm_tree_plan_stats(const octofmm_tree_plan* plan, octofmm_tree_stats* stats);
 /* Runs each phase `iters` times between HIP events on `stream`; ms[0] upward
- * pass, ms[1] interactions (all levels), ms[2] downward pass and assembly. */
+ * pass, ms[1] interactions of all levels with the downward pass and the leaf
+ * force assembly fused into their kernels, ms[2] = 0 (kept for layout). */
 octofmm_status octofmm_tree_plan_time_phases(octofmm_tree_plan* plan, void* stream, int32_t iters,
     double* ms);
 octofmm_status octofmm_tree_plan_destroy(octofmm_tree_plan* plan);

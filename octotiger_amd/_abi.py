@@ -68,8 +68,11 @@ class Config(ctypes.Structure):
     _fields_ = [
         ("theta", ctypes.c_double),
         ("device", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
     ]
+
+
+CHECK_FINITE = 1
 
 
 class Level(ctypes.Structure):

@@ -416,6 +416,20 @@ struct LevelArgs {
     const int32_t* p2m_dir;
     const int4* p2m_tgt;
     const int32_t* p2m_part;
+    // Whole-tree solves (l2l != 0): the level's expansions are written, not
+    // accumulated, as interactions + the parent cell's final expansion
+    // re-expanded about the cell's centre of mass (and L_c + the parent's
+    // L_c); leaf cells also get phi and g.  parent / octant per slot, pL /
+    // pLc / pcom of the level above (null on level 0).
+    const int32_t* parent;
+    const int32_t* octant;
+    const double* pL;
+    const double* pLc;
+    const double* pcom;
+    double* g;
+    double G;
+    int32_t l2l;
+    int32_t correct_am;
     int32_t nitems;
     int32_t item0;          // first item of this launch
     double inv_dx;
@@ -436,6 +450,7 @@ struct RootReduceArgs {
     double* Lc;             // [512][3]
     int32_t nchunks;
     int32_t rho;
+    int32_t overwrite;      // whole-tree solve: L is written, not accumulated
 };
 
 // ---------------------------------------------------------------------------
@@ -534,21 +549,5 @@ struct M2MArgs {
     int32_t rho;
 };
 
-struct L2LArgs {
-    double* L;              // [n][512][20] this level
-    double* Lc;             // [n][512][3]
-    const double* com;      // [n][512][3]
-    const int32_t* kind;
-    const int32_t* parent;
-    const int32_t* octant;
-    const double* pL;       // parent level
-    const double* pLc;
-    const double* pcom;
-    double* g;              // [n][512][4] or null
-    double G;
-    int32_t nnodes;
-    int32_t rho;
-    int32_t correct_am;
-};
 
 }   // namespace octofmm

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstdio>
@@ -125,6 +126,72 @@ __device__ __forceinline__ int partner_base(const LaneGeom& g, int ax, int ay, i
     return ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
 }
 
+// Result of one target cell.  Level plans accumulate L (ncomp components:
+// 20 for M2L targets, 4 for leaves) and overwrite L_c when the kernel forms
+// it (write_c).  In a whole-tree solve the cell's final expansion is written
+// directly: interactions + the parent cell's expansion re-expanded about the
+// cell's centre of mass (taylor::operator<<=, grid::compute_expansions,
+// src/grid_fmm.cpp:1524-1637), L_c + the parent's L_c, and for leaf cells
+// phi = G L0 and g = -G (L_1..3 + L_c).
+template <bool RHO>
+__device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int cell, const double* L, int ncomp,
+    const double* C, bool write_c, bool leaf_target) {
+    double* Lo = a.L + ((size_t)slot * kCells + cell) * 20;
+    double* Co = a.Lc + ((size_t)slot * kCells + cell) * 3;
+    if (!a.l2l) {
+        for (int i = 0; i < ncomp; ++i) Lo[i] += L[i];
+        if (RHO && write_c) {
+            Co[0] = C[0];
+            Co[1] = C[1];
+            Co[2] = C[2];
+        }
+        return;
+    }
+    double l[20], lc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 20; ++j) l[j] = 0.0;
+    const int pn = a.parent[slot];
+    if (pn >= 0) {
+        const int pc = parent_cell(a.octant[slot], cell >> 7, (cell >> 4) & 3, (cell >> 1) & 3);
+        double A[20];
+        const double* src = a.pL + ((size_t)pn * kCells + pc) * 20;
+#pragma unroll
+        for (int j = 0; j < 20; ++j) A[j] = src[j];
+        const double* Y = a.pcom + ((size_t)pn * kCells + pc) * 3;
+        const double* X = a.com + ((size_t)slot * kCells + cell) * 3;
+        const double dX[3] = {X[0] - Y[0], X[1] - Y[1], X[2] - Y[2]};
+        local_shift(A, dX, l);
+        if (RHO) {
+            const double* q = a.pLc + ((size_t)pn * kCells + pc) * 3;
+            lc[0] = q[0];
+            lc[1] = q[1];
+            lc[2] = q[2];
+        }
+    }
+    double Lf[4];
+#pragma unroll
+    for (int j = 0; j < 20; ++j) {
+        const double v = (j < ncomp ? L[j] : 0.0) + l[j];
+        Lo[j] = v;
+        if (j < 4) Lf[j] = v;
+    }
+    double Cf[3] = {0.0, 0.0, 0.0};
+    if (RHO) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            Cf[d] = (write_c ? C[d] : 0.0) + lc[d];
+            Co[d] = Cf[d];
+        }
+    }
+    if (a.g && leaf_target) {
+        double* go = a.g + ((size_t)slot * kCells + cell) * 4;
+        go[0] = a.G * Lf[0];
+        if (RHO)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) go[1 + d] = -a.G * Lf[1 + d] - (a.correct_am ? a.G * Cf[d] : 0.0);
+    }
+}
+
 template <bool RHO, bool MIXED>
 __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind, int w) {
     const LaneGeom g = lane_geom(w);
@@ -216,15 +283,7 @@ __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_n
             }
         }
     }
-    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
-#pragma unroll
-    for (int i = 0; i < 20; ++i) Lo[i] += L[i];
-    if (RHO) {
-        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
-        Co[0] = C[0];
-        Co[1] = C[1];
-        Co[2] = C[2];
-    }
+    finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
 }
 
 // Waits for this wave's LDS writes (lgkmcnt(0), global loads left in flight)
@@ -337,15 +396,7 @@ __device__ void m2l_item_lds(const LevelArgs& a, int slot, int flags, const int*
             }
         }
     }
-    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
-#pragma unroll
-    for (int i = 0; i < 20; ++i) Lo[i] += L[i];
-    if (RHO) {
-        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
-        Co[0] = C[0];
-        Co[1] = C[1];
-        Co[2] = C[2];
-    }
+    finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
 }
 
 // Mapping 3: mapping 2 with the staging done by direct global->LDS loads
@@ -451,15 +502,7 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             }
         }
     }
-    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
-#pragma unroll
-    for (int i = 0; i < 20; ++i) Lo[i] += L[i];
-    if (RHO) {
-        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
-        Co[0] = C[0];
-        Co[1] = C[1];
-        Co[2] = C[2];
-    }
+    finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
 }
 
 // Mapping 4: mapping 3 with the image split by child halves and double
@@ -585,15 +628,7 @@ __device__ void m2l_item_direct2(const LevelArgs& a, int slot, int flags, const 
             }
         }
     }
-    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
-#pragma unroll
-    for (int i = 0; i < 20; ++i) Lo[i] += L[i];
-    if (RHO) {
-        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
-        Co[0] = C[0];
-        Co[1] = C[1];
-        Co[2] = C[2];
-    }
+    finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
 }
 
 // Mapping 1: wave = child class k over the 64 coarse cells of the sub-grid;
@@ -663,15 +698,7 @@ __device__ void m2l_item_classwave(const LevelArgs& a, int slot, int flags, cons
             }
         }
     }
-    double* Lo = a.L + ((size_t)slot * kCells + g.cell) * 20;
-#pragma unroll
-    for (int i = 0; i < 20; ++i) Lo[i] += L[i];
-    if (RHO) {
-        double* Co = a.Lc + ((size_t)slot * kCells + g.cell) * 3;
-        Co[0] = C[0];
-        Co[1] = C[1];
-        Co[2] = C[2];
-    }
+    finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
 }
 
 // Leaf targets (monopole kernels): one 512-lane workgroup per leaf sub-grid.
@@ -719,11 +746,12 @@ __global__ __launch_bounds__(512) void leaf_kernel(LevelArgs a) {
     const double* wp = s_win + ((tx + h) * W + (ty + h)) * W + (tz + h) - 2048;
     double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
     const int32_t* __restrict__ list = tab->p2plist[k];
+    const double(*__restrict__ fl)[4] = tab->p2pfour[k];
     const int n = tab->np2p_k[k];
-#pragma unroll 4
+#pragma unroll 8
     for (int q = 0; q < n; ++q) {
         const int e = list[q];
-        const double* f = tab->four[e & 0xffff];
+        const double* f = fl[q];
         const double m = wp[e >> 16];
         p0 = fma(f[0], m, p0);
         p1 = fma(f[1], m, p1);
@@ -779,15 +807,7 @@ __global__ __launch_bounds__(512) void leaf_kernel(LevelArgs a) {
         C[1] = o[5];
         C[2] = o[6];
     }
-    double* Lo = a.L + ((size_t)slot * kCells + cell) * 20;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) Lo[i] += L[i];
-    if (RHO && p2m) {
-        double* Co = a.Lc + ((size_t)slot * kCells + cell) * 3;
-        Co[0] = C[0];
-        Co[1] = C[1];
-        Co[2] = C[2];
-    }
+    finish_cell<RHO>(a, slot, cell, L, 4, C, p2m, true);
 }
 
 template <bool RHO>
@@ -922,7 +942,7 @@ __global__ __launch_bounds__(256) void root_reduce(RootReduceArgs a) {
     double s = 0.0;
     for (int q = 0; q < a.nchunks; ++q) s += a.partial[((size_t)q * kCells + t) * kRecDoubles + c];
     if (c < 20)
-        a.L[(size_t)t * 20 + c] += s;
+        a.L[(size_t)t * 20 + c] = a.overwrite ? s : a.L[(size_t)t * 20 + c] + s;
     else if (a.rho)
         a.Lc[(size_t)t * 3 + (c - 20)] = s;
 }
@@ -940,138 +960,78 @@ __device__ __forceinline__ double cell_centre(double xb, int i, double dx) {
 }
 
 // Upward pass of one level (grid::compute_multipoles, src/grid_fmm.cpp:1639-1803):
-// one thread per coarse cell (64 per node).  For RHO the leaf nodes' cell
-// centres are written first and every coarse cell's mass-weighted centre goes
-// to the parent; the 8 children's multipoles are shifted to it and summed
-// into the parent's cell.
+// one lane per child cell, 8 consecutive lanes per coarse cell (512 per
+// node).  For RHO the leaf nodes' cell centres are written first; the 8 lanes
+// form the coarse cell's mass and mass-weighted centre by a fixed butterfly,
+// shift their child's multipole to it, and sum the shifted multipoles the
+// same way; lane 0 of the group writes the parent's cell.
+__device__ __forceinline__ double sum8(double v) {
+    v += __shfl_xor(v, 1, 8);
+    v += __shfl_xor(v, 2, 8);
+    v += __shfl_xor(v, 4, 8);
+    return v;
+}
+
 __global__ __launch_bounds__(256) void m2m_kernel(M2MArgs a) {
     const int gid = blockIdx.x * 256 + threadIdx.x;
-    if (gid >= a.nnodes * 64) return;
-    const int n = gid >> 6, q = gid & 63;
+    if (gid >= a.nnodes * kCells) return;     // whole 8-lane groups exit together
+    const int n = gid >> 9, q = (gid >> 3) & 63, ci = gid & 7;
     const int ip = q >> 4, jp = (q >> 2) & 3, kp = q & 3;
     const bool leaf = a.kind[n] == OCTOFMM_SLOT_LEAF;
-    double* c0 = a.com + (size_t)n * kCells * 3;
-    int u[8];
-#pragma unroll
-    for (int ci = 0; ci < 8; ++ci)
-        u[ci] = (2 * ip + (ci & 1)) * 64 + (2 * jp + ((ci >> 1) & 1)) * 8 + (2 * kp + ((ci >> 2) & 1));
+    const int u = (2 * ip + (ci & 1)) * 64 + (2 * jp + ((ci >> 1) & 1)) * 8 + (2 * kp + ((ci >> 2) & 1));
+    double* X = a.com + ((size_t)n * kCells + u) * 3;
     if (leaf && a.rho) {
         const double* xb = a.xbase + 3 * n;
-#pragma unroll
-        for (int ci = 0; ci < 8; ++ci) {
-            double* p = c0 + 3 * u[ci];
-            p[0] = cell_centre(xb[0], u[ci] >> 6, a.dx);
-            p[1] = cell_centre(xb[1], (u[ci] >> 3) & 7, a.dx);
-            p[2] = cell_centre(xb[2], u[ci] & 7, a.dx);
-        }
+        X[0] = cell_centre(xb[0], u >> 6, a.dx);
+        X[1] = cell_centre(xb[1], (u >> 3) & 7, a.dx);
+        X[2] = cell_centre(xb[2], u & 7, a.dx);
     }
     const int pn = a.parent[n];
-    if (pn < 0) return;
+    if (pn < 0) return;                        // uniform per node
     const int pc = parent_cell(a.octant[n], ip, jp, kp);
     double* Yp = a.pcom + ((size_t)pn * kCells + pc) * 3;
+    double A[20];
+    if (leaf) {
+        A[0] = a.mon[(size_t)n * kCells + u];
+#pragma unroll
+        for (int j = 1; j < 20; ++j) A[j] = 0.0;
+    } else {
+        const double2* src = reinterpret_cast<const double2*>(a.M + ((size_t)n * kCells + u) * 20);
+#pragma unroll
+        for (int j = 0; j < 10; ++j) {
+            const double2 v = src[j];
+            A[2 * j] = v.x;
+            A[2 * j + 1] = v.y;
+        }
+    }
+    const double x0 = X[0], x1 = X[1], x2 = X[2];
     double Y[3];
     if (a.rho) {
-        double mtot = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll
-        for (int ci = 0; ci < 8; ++ci) {
-            const double m = leaf ? a.mon[(size_t)n * kCells + u[ci]] : a.M[((size_t)n * kCells + u[ci]) * 20];
-            const double* X = c0 + 3 * u[ci];
-            mtot += m;
-            s0 += X[0] * m;
-            s1 += X[1] * m;
-            s2 += X[2] * m;
+        const double mtot = sum8(A[0]);
+        Y[0] = sum8(x0 * A[0]) / mtot;
+        Y[1] = sum8(x1 * A[0]) / mtot;
+        Y[2] = sum8(x2 * A[0]) / mtot;
+        if (ci == 0) {
+            Yp[0] = Y[0];
+            Yp[1] = Y[1];
+            Yp[2] = Y[2];
         }
-        Y[0] = s0 / mtot;
-        Y[1] = s1 / mtot;
-        Y[2] = s2 / mtot;
-        Yp[0] = Y[0];
-        Yp[1] = Y[1];
-        Yp[2] = Y[2];
     } else {
         Y[0] = Yp[0];
         Y[1] = Yp[1];
         Y[2] = Yp[2];
     }
-    double MM[20];
-#pragma unroll
-    for (int j = 0; j < 20; ++j) MM[j] = 0.0;
-    for (int ci = 0; ci < 8; ++ci) {
-        double A[20], B[20];
-        if (leaf) {
-            A[0] = a.mon[(size_t)n * kCells + u[ci]];
-#pragma unroll
-            for (int j = 1; j < 20; ++j) A[j] = 0.0;
-        } else {
-            const double2* src = reinterpret_cast<const double2*>(a.M + ((size_t)n * kCells + u[ci]) * 20);
-#pragma unroll
-            for (int j = 0; j < 10; ++j) {
-                const double2 v = src[j];
-                A[2 * j] = v.x;
-                A[2 * j + 1] = v.y;
-            }
-        }
-        const double* X = c0 + 3 * u[ci];
-        const double dX[3] = {X[0] - Y[0], X[1] - Y[1], X[2] - Y[2]};
-        multipole_shift(A, dX, B);
-#pragma unroll
-        for (int j = 0; j < 20; ++j) MM[j] += B[j];
-    }
+    const double dX[3] = {x0 - Y[0], x1 - Y[1], x2 - Y[2]};
+    double B[20];
+    multipole_shift(A, dX, B);
     double* Mo = a.pM + ((size_t)pn * kCells + pc) * 20;
 #pragma unroll
-    for (int j = 0; j < 20; ++j) Mo[j] = MM[j];
+    for (int j = 0; j < 20; ++j) {
+        const double v = sum8(B[j]);
+        if (ci == 0) Mo[j] = v;
+    }
 }
 
-// Downward pass of one level with the leaf force assembly
-// (grid::compute_expansions, src/grid_fmm.cpp:1524-1637): one thread per cell.
-__global__ __launch_bounds__(256) void l2l_kernel(L2LArgs a) {
-    const int gid = blockIdx.x * 256 + threadIdx.x;
-    if (gid >= a.nnodes * kCells) return;
-    const int n = gid >> 9, u = gid & 511;
-    double l[20], lc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int j = 0; j < 20; ++j) l[j] = 0.0;
-    const int pn = a.parent[n];
-    if (pn >= 0) {
-        const int pc = parent_cell(a.octant[n], u >> 7, (u >> 4) & 3, (u >> 1) & 3);
-        double A[20];
-        const double* src = a.pL + ((size_t)pn * kCells + pc) * 20;
-#pragma unroll
-        for (int j = 0; j < 20; ++j) A[j] = src[j];
-        const double* Y = a.pcom + ((size_t)pn * kCells + pc) * 3;
-        const double* X = a.com + ((size_t)n * kCells + u) * 3;
-        const double dX[3] = {X[0] - Y[0], X[1] - Y[1], X[2] - Y[2]};
-        local_shift(A, dX, l);
-        if (a.rho) {
-            const double* s = a.pLc + ((size_t)pn * kCells + pc) * 3;
-            lc[0] = s[0];
-            lc[1] = s[1];
-            lc[2] = s[2];
-        }
-    }
-    double* Lo = a.L + ((size_t)n * kCells + u) * 20;
-    double Lf[20];
-#pragma unroll
-    for (int j = 0; j < 20; ++j) {
-        Lf[j] = Lo[j] + l[j];
-        Lo[j] = Lf[j];
-    }
-    double C[3] = {0.0, 0.0, 0.0};
-    if (a.rho) {
-        double* Co = a.Lc + ((size_t)n * kCells + u) * 3;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            C[d] = Co[d] + lc[d];
-            Co[d] = C[d];
-        }
-    }
-    if (a.g && a.kind[n] == OCTOFMM_SLOT_LEAF) {
-        double* go = a.g + ((size_t)n * kCells + u) * 4;
-        go[0] = a.G * Lf[0];
-        if (a.rho)
-#pragma unroll
-            for (int d = 0; d < 3; ++d) go[1 + d] = -a.G * Lf[1 + d] - (a.correct_am ? a.G * C[d] : 0.0);
-    }
-}
 
 }   // namespace
 
@@ -1081,6 +1041,7 @@ __global__ __launch_bounds__(256) void l2l_kernel(L2LArgs a) {
 struct octofmm_context {
     int device = 0;
     double theta = 0.5;
+    int flags = 0;                 // OCTOFMM_CHECK_FINITE
     Tables tab;
     Tables* d_tab = nullptr;
     hipStream_t stream = nullptr;
@@ -1130,6 +1091,16 @@ struct octofmm_plan {
     int variant = 8;           // level-kernel variant (see kVariants)
     uint64_t* d_mixbits = nullptr;
     bool root = false;             // root sub-grid: root stencil over slot 0
+    // set by a whole-tree plan: write final expansions (fused downward pass)
+    bool l2l = false;
+    const int32_t* t_parent = nullptr;
+    const int32_t* t_octant = nullptr;
+    const double* t_pL = nullptr;
+    const double* t_pLc = nullptr;
+    const double* t_pcom = nullptr;
+    double* t_g = nullptr;
+    double t_G = 1.0;
+    int32_t t_correct_am = 1;
     double* d_partial = nullptr;   // root: per-chunk partial expansions
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1200,6 +1171,35 @@ bool check_neighbor(const octofmm_neighbor& nb, bool want_multipole) {
             if (nb.cells[i] < 0 || nb.cells[i] >= kCells) return false;
     (void)want_multipole;
     return true;
+}
+
+bool all_finite(const double* p, size_t n) {
+    if (!p) return true;
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+// OCTOFMM_CHECK_FINITE: the sub-grid's own arrays and every neighbour's.
+octofmm_status check_finite_inputs(octofmm_context* c, const octofmm_subgrid* in, bool leaf_target) {
+    if (!(c->flags & OCTOFMM_CHECK_FINITE)) return OCTOFMM_OK;
+    if (!leaf_target && !all_finite(in->M, (size_t)kCells * 20))
+        return fail(c, OCTOFMM_ERR_NONFINITE, "non-finite multipole in the sub-grid");
+    if (!all_finite(in->com, (size_t)kCells * 3))
+        return fail(c, OCTOFMM_ERR_NONFINITE, "non-finite centre of mass in the sub-grid");
+    if (leaf_target && !all_finite(in->mon, kCells))
+        return fail(c, OCTOFMM_ERR_NONFINITE, "non-finite monopole in the sub-grid");
+    if (!std::isfinite(in->dx) || !std::isfinite(in->xbase[0]) || !std::isfinite(in->xbase[1]) ||
+        !std::isfinite(in->xbase[2]))
+        return fail(c, OCTOFMM_ERR_NONFINITE, "non-finite dx or xbase");
+    for (int f = 0; f < 26; ++f) {
+        const octofmm_neighbor& nb = in->neighbors[f];
+        if (nb.kind == OCTOFMM_NEIGHBOR_EMPTY) continue;
+        const size_t n = (size_t)nb.count;
+        if (!all_finite(nb.M, n * 20) || !all_finite(nb.com, n * 3) || !all_finite(nb.mon, n))
+            return fail(c, OCTOFMM_ERR_NONFINITE, "non-finite data in neighbour %d", f);
+    }
+    return OCTOFMM_OK;
 }
 
 // Fills the 27-slot staging buffers from a sub-grid descriptor.  Partial
@@ -1379,6 +1379,7 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         rr.Lc = C13;
         rr.nchunks = kRootChunks;
         rr.rho = rho;
+        rr.overwrite = 0;
         hipLaunchKernelGGL(root_reduce, dim3((kCells * 23 + 255) / 256), dim3(256), 0, st, rr);
         HIPC(c, hipGetLastError());
     } else {
@@ -1396,6 +1397,13 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         la.p2m_dir = c->d_p2m_dir;
         la.p2m_tgt = c->d_p2m_tgt;
         la.p2m_part = c->d_p2m_part;
+        la.parent = nullptr;
+        la.octant = nullptr;
+        la.pL = la.pLc = la.pcom = nullptr;
+        la.g = nullptr;
+        la.G = 1.0;
+        la.l2l = 0;
+        la.correct_am = 0;
         la.nitems = 1;
         la.item0 = 0;
         la.inv_dx = 1.0 / dx;
@@ -1451,6 +1459,7 @@ octofmm_status octofmm_create(const octofmm_config* cfg, octofmm_context** out) 
     octofmm_context* c = new octofmm_context();
     c->device = cfg->device;
     c->theta = cfg->theta;
+    c->flags = cfg->flags;
     char err[256];
     if (!build_tables(cfg->theta, c->tab, err, sizeof(err))) {
         delete c;
@@ -1524,8 +1533,10 @@ octofmm_status octofmm_compute_multipole_interactions(octofmm_context* c, const 
                 return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad neighbour %d", f);
             if (nb.kind == OCTOFMM_NEIGHBOR_MONOPOLE && nb.mon) flags |= kItemNear;
         }
+    octofmm_status s = check_finite_inputs(c, in, false);
+    if (s != OCTOFMM_OK) return s;
     HIPC(c, hipSetDevice(c->device));
-    octofmm_status s = ensure_workspace(c);
+    s = ensure_workspace(c);
     if (s != OCTOFMM_OK) return s;
     s = fill_workspace(c, in, false, root);
     if (s != OCTOFMM_OK) return s;
@@ -1551,8 +1562,10 @@ octofmm_status octofmm_compute_monopole_interactions(octofmm_context* c, const o
         if (nb.kind != OCTOFMM_NEIGHBOR_EMPTY && !check_neighbor(nb, nb.kind == OCTOFMM_NEIGHBOR_MULTIPOLE))
             return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad neighbour %d", f);
     }
+    octofmm_status s = check_finite_inputs(c, in, true);
+    if (s != OCTOFMM_OK) return s;
     HIPC(c, hipSetDevice(c->device));
-    octofmm_status s = ensure_workspace(c);
+    s = ensure_workspace(c);
     if (s != OCTOFMM_OK) return s;
     s = fill_workspace(c, in, true, false);
     if (s != OCTOFMM_OK) return s;
@@ -1623,6 +1636,7 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         rr.Lc = p->d_Lc;
         rr.nchunks = kRootChunks;
         rr.rho = p->type == OCTOFMM_RHO;
+        rr.overwrite = p->l2l ? 1 : 0;
         hipLaunchKernelGGL(root_reduce, dim3((kCells * 23 + 255) / 256), dim3(256), 0, st, rr);
         HIPC(c, hipGetLastError());
     } else if ((stage_mask & 2) && p->nitems > 0) {
@@ -1640,6 +1654,15 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         la.p2m_dir = c->d_p2m_dir;
         la.p2m_tgt = c->d_p2m_tgt;
         la.p2m_part = c->d_p2m_part;
+        la.parent = p->t_parent;
+        la.octant = p->t_octant;
+        la.pL = p->t_pL;
+        la.pLc = p->t_pLc;
+        la.pcom = p->t_pcom;
+        la.g = p->t_g;
+        la.G = p->t_G;
+        la.l2l = p->l2l ? 1 : 0;
+        la.correct_am = p->t_correct_am;
         la.inv_dx = 1.0 / p->dx;
         la.neg_inv_dx = -1.0 / p->dx;
         const bool rho = p->type == OCTOFMM_RHO;
@@ -2065,7 +2088,7 @@ void tree_free(octofmm_tree_plan* t) {
     delete t;
 }
 
-// phase bit 0: upward pass; bit 1: clear outputs + interactions; bit 2: downward pass
+// phase bit 0: upward pass; bit 1: interactions with the fused downward pass
 octofmm_status tree_launch(octofmm_tree_plan* t, hipStream_t st, int phases) {
     octofmm_context* c = t->ctx;
     const int nl = (int)t->lv.size();
@@ -2086,42 +2109,16 @@ octofmm_status tree_launch(octofmm_tree_plan* t, hipStream_t st, int phases) {
             a.dx = L.dx;
             a.nnodes = L.nnodes;
             a.rho = rho;
-            hipLaunchKernelGGL(m2m_kernel, dim3((L.nnodes * 64 + 255) / 256), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(m2m_kernel, dim3((L.nnodes * kCells + 255) / 256), dim3(256), 0, st, a);
             HIPC(c, hipGetLastError());
         }
-    if (phases & 2) {
-        for (int l = 0; l < nl; ++l) {
-            const auto& L = t->lv[l];
-            const size_t nL = (size_t)L.nnodes * kCells * 20, nC = (size_t)L.nnodes * kCells * 3;
-            hipLaunchKernelGGL(zero_fill, dim3((unsigned)((nL + 255) / 256)), dim3(256), 0, st, L.L, nL);
-            if (rho) hipLaunchKernelGGL(zero_fill, dim3((unsigned)((nC + 255) / 256)), dim3(256), 0, st, L.Lc, nC);
-            HIPC(c, hipGetLastError());
-        }
+    // interactions of every level, root first; each level's kernels also
+    // write the final expansions (the downward pass is fused into them), so
+    // level l reads the finished level l - 1
+    if (phases & 2)
         for (int l = 0; l < nl; ++l) {
             const octofmm_status s = plan_launch(t->lv[l].plan, st, 3);
             if (s != OCTOFMM_OK) return s;
-        }
-    }
-    if (phases & 4)
-        for (int l = 0; l < nl; ++l) {
-            const auto& L = t->lv[l];
-            L2LArgs a;
-            a.L = L.L;
-            a.Lc = L.Lc;
-            a.com = L.com;
-            a.kind = L.d_kind;
-            a.parent = L.d_parent;
-            a.octant = L.d_octant;
-            a.pL = l > 0 ? t->lv[l - 1].L : nullptr;
-            a.pLc = l > 0 ? t->lv[l - 1].Lc : nullptr;
-            a.pcom = l > 0 ? t->lv[l - 1].com : nullptr;
-            a.g = L.g;
-            a.G = t->G;
-            a.nnodes = L.nnodes;
-            a.rho = rho;
-            a.correct_am = t->correct_am;
-            hipLaunchKernelGGL(l2l_kernel, dim3((L.nnodes * kCells + 255) / 256), dim3(256), 0, st, a);
-            HIPC(c, hipGetLastError());
         }
     return OCTOFMM_OK;
 }
@@ -2255,6 +2252,16 @@ octofmm_status octofmm_tree_plan_create(octofmm_context* c, const octofmm_tree* 
             tree_free(t);
             return fail(c, s, "level %d: %s", l, msg.c_str());
         }
+        octofmm_plan* lp = t->lv[l].plan;
+        lp->l2l = true;
+        lp->t_parent = t->lv[l].d_parent;
+        lp->t_octant = t->lv[l].d_octant;
+        lp->t_pL = l > 0 ? t->lv[l - 1].L : nullptr;
+        lp->t_pLc = l > 0 ? t->lv[l - 1].Lc : nullptr;
+        lp->t_pcom = l > 0 ? t->lv[l - 1].com : nullptr;
+        lp->t_g = t->lv[l].g;
+        lp->t_G = t->G;
+        lp->t_correct_am = t->correct_am;
         const octofmm_plan_stats& ps = t->lv[l].plan->stats;
         if (t->lv[l].plan->root)
             t->stats.root_pairs += ps.far_pairs;

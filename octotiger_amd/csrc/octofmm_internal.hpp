@@ -55,6 +55,9 @@ struct Tables {
     int32_t p2p_halo;
     int32_t np2p_k[8];
     int32_t p2plist[8][kMaxDelta * 8];
+    // the four constants of p2plist[k][q], in the same order, so the kernel
+    // loads both without one load waiting for the other
+    double p2pfour[8][kMaxDelta * 8][4];
 };
 
 // Level-kernel variants: lane mapping x 100 + waves per workgroup.

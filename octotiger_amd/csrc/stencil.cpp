@@ -160,7 +160,9 @@ bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
                 const int dy = 2 * t.delta[i][1] + ((j >> 1) & 1) - ((k >> 1) & 1);
                 const int dz = 2 * t.delta[i][2] + (j & 1) - (k & 1);
                 const int woff = (dx * W + dy) * W + dz;
-                t.p2plist[k][q++] = box_index(dx, dy, dz) | ((woff + 2048) << 16);
+                const int bi = box_index(dx, dy, dz);
+                for (int c = 0; c < 4; ++c) t.p2pfour[k][q][c] = t.four[bi][c];
+                t.p2plist[k][q++] = bi | ((woff + 2048) << 16);
             }
         t.np2p_k[k] = q;
     }

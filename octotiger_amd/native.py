@@ -72,9 +72,9 @@ def lib():
 class Context:
     """One GPU, one opening angle; stencils are built at creation."""
 
-    def __init__(self, theta=0.5, device=0):
+    def __init__(self, theta=0.5, device=0, check_finite=False):
         self._h = ctypes.c_void_p()
-        cfg = _abi.Config(theta=theta, device=device)
+        cfg = _abi.Config(theta=theta, device=device, flags=_abi.CHECK_FINITE if check_finite else 0)
         st = lib().octofmm_create(ctypes.byref(cfg), ctypes.byref(self._h))
         if st != _abi.OK:
             raise OctofmmError(st, lib().octofmm_status_string(st).decode())

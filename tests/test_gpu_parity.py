@@ -243,3 +243,20 @@ def test_level_config3_full_size(ctx):
     mass = np.where(lv.kind[tg, None] == _abi.SLOT_LEAF, lv.mon[tg], lv.M[tg, :, 0])
     ef, et = force_torque_error(L[tg], Lc[tg], Lr[tg], Lcr[tg], mass, lv.com[tg])
     assert ef <= TOL and et <= TOL
+
+
+def test_dropin_checks_finiteness_when_asked():
+    """With OCTOFMM_CHECK_FINITE the drop-in rejects a NaN in a neighbour's
+    multipoles with ERR_NONFINITE before any device work; clean inputs pass."""
+    c = native.Context(theta=0.5, device=0, check_finite=True)
+    sg, keep, kind, _ = fmm_cases.subgrid("mixed_halo", _abi.RHO)
+    L = np.zeros((512, 20))
+    Lc = np.zeros((512, 3))
+    c.multipole_interactions(sg, L, Lc)                       # clean: accepted
+    f = next(i for i in range(26) if sg.neighbors[i].kind == _abi.NEIGHBOR_MULTIPOLE)
+    sg.neighbors[f].M[7] = float("nan")
+    L2 = np.zeros((512, 20))
+    with pytest.raises(native.OctofmmError) as e:
+        c.multipole_interactions(sg, L2, Lc)
+    assert e.value.status == _abi.ERR_NONFINITE and not L2.any()
+    c.close()
