@@ -136,10 +136,16 @@ __device__ __forceinline__ int partner_base(const LaneGeom& g, int ax, int ay, i
 template <bool RHO>
 __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int cell, const double* L, int ncomp,
     const double* C, bool write_c, bool leaf_target) {
-    double* Lo = a.L + ((size_t)slot * kCells + cell) * 20;
+    // expansion rows are 160 bytes and 16-byte aligned: moved as 10 double2
+    double2* Lo = reinterpret_cast<double2*>(a.L + ((size_t)slot * kCells + cell) * 20);
     double* Co = a.Lc + ((size_t)slot * kCells + cell) * 3;
     if (!a.l2l) {
-        for (int i = 0; i < ncomp; ++i) Lo[i] += L[i];
+        for (int i = 0; i < ncomp / 2; ++i) {
+            double2 v = Lo[i];
+            v.x += L[2 * i];
+            v.y += L[2 * i + 1];
+            Lo[i] = v;
+        }
         if (RHO && write_c) {
             Co[0] = C[0];
             Co[1] = C[1];
@@ -154,9 +160,13 @@ __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int ce
     if (pn >= 0) {
         const int pc = parent_cell(a.octant[slot], cell >> 7, (cell >> 4) & 3, (cell >> 1) & 3);
         double A[20];
-        const double* src = a.pL + ((size_t)pn * kCells + pc) * 20;
+        const double2* src = reinterpret_cast<const double2*>(a.pL + ((size_t)pn * kCells + pc) * 20);
 #pragma unroll
-        for (int j = 0; j < 20; ++j) A[j] = src[j];
+        for (int j = 0; j < 10; ++j) {
+            const double2 v = src[j];
+            A[2 * j] = v.x;
+            A[2 * j + 1] = v.y;
+        }
         const double* Y = a.pcom + ((size_t)pn * kCells + pc) * 3;
         const double* X = a.com + ((size_t)slot * kCells + cell) * 3;
         const double dX[3] = {X[0] - Y[0], X[1] - Y[1], X[2] - Y[2]};
@@ -168,13 +178,11 @@ __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int ce
             lc[2] = q[2];
         }
     }
-    double Lf[4];
+    double Lf[20];
 #pragma unroll
-    for (int j = 0; j < 20; ++j) {
-        const double v = (j < ncomp ? L[j] : 0.0) + l[j];
-        Lo[j] = v;
-        if (j < 4) Lf[j] = v;
-    }
+    for (int j = 0; j < 20; ++j) Lf[j] = (j < ncomp ? L[j] : 0.0) + l[j];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) Lo[j] = make_double2(Lf[2 * j], Lf[2 * j + 1]);
     double Cf[3] = {0.0, 0.0, 0.0};
     if (RHO) {
 #pragma unroll
@@ -184,11 +192,17 @@ __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int ce
         }
     }
     if (a.g && leaf_target) {
-        double* go = a.g + ((size_t)slot * kCells + cell) * 4;
-        go[0] = a.G * Lf[0];
+        double2* go = reinterpret_cast<double2*>(a.g + ((size_t)slot * kCells + cell) * 4);
+        double g4[4] = {a.G * Lf[0], 0.0, 0.0, 0.0};
         if (RHO)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) go[1 + d] = -a.G * Lf[1 + d] - (a.correct_am ? a.G * Cf[d] : 0.0);
+            for (int d = 0; d < 3; ++d) g4[1 + d] = -a.G * Lf[1 + d] - (a.correct_am ? a.G * Cf[d] : 0.0);
+        if (RHO) {
+            go[0] = make_double2(g4[0], g4[1]);
+            go[1] = make_double2(g4[2], g4[3]);
+        } else {
+            reinterpret_cast<double*>(go)[0] = g4[0];   // DRHODT: phi slot only
+        }
     }
 }
 
@@ -713,7 +727,7 @@ __device__ void m2l_item_classwave(const LevelArgs& a, int slot, int flags, cons
 // neighbours then contribute through p2m with their full multipoles over the
 // same stencil, far and near alike (monopole_kernel_templates.hpp:69-239).
 template <bool RHO>
-__global__ __launch_bounds__(512) void leaf_kernel(LevelArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void leaf_kernel(LevelArgs a) {
     extern __shared__ double s_win[];
     __shared__ int s_nbr[27];
     __shared__ int s_kind[27];
