@@ -300,6 +300,98 @@ __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_n
     finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
 }
 
+// Mapping 5: the coarse offsets of one 2x2x2 coarse block are split over
+// kSplit waves of a workgroup (wave s takes offsets s, s + kSplit, ...), so a
+// sub-grid runs on 8 * kSplit waves; this shortens the serial time of one
+// item for levels with few sub-grids.  Waves s > 0 leave their partial
+// expansions in LDS and wave 0 adds them in the order s = 1, 2, ... before
+// writing the result.
+constexpr int kSplit = 4;
+
+template <bool RHO>
+__device__ void m2l_item_split(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind, int w,
+    int sl, double* __restrict__ s_part) {
+    const LaneGeom g = lane_geom(w);
+    const int lane = threadIdx.x & 63;
+    const Tables* __restrict__ tab = a.tab;
+    const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
+    const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
+    const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
+    double mc[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
+    double L[20], C[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 20; ++i) L[i] = 0.0;
+    const int nd = tab->ndelta;
+    for (int di = sl; di < nd; di += kSplit) {
+        const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+        unsigned fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
+        int ns, raw;
+        const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
+        if (ns < 0) fb = 0;
+        const double* pb = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
+        Rec nxt;
+        load_rec(pb, nxt);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const Rec cur = nxt;
+            if (j < 7) load_rec(pb + (j + 1) * kRecDoubles, nxt);
+            if (fb & (1u << j)) m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
+        }
+        if (flags & kItemNear) {
+            unsigned nb = (unsigned)(tab->nearbits[di] >> (8 * g.k)) & 0xffu;
+            if (ns < 0 || s_kind[raw] != OCTOFMM_SLOT_LEAF) nb = 0;
+#pragma unroll 1
+            for (int j = 0; j < 8; ++j) {
+                if (nb & (1u << j)) {
+                    const double* p = pb + j * kRecDoubles;
+                    m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, inv_mc0, mc, L, C);
+                }
+            }
+        }
+    }
+    if (sl > 0) {
+        double* o = s_part + ((size_t)(sl - 1) * 64 + lane) * 23;
+#pragma unroll
+        for (int i = 0; i < 20; ++i) o[i] = L[i];
+        o[20] = C[0];
+        o[21] = C[1];
+        o[22] = C[2];
+    }
+    __syncthreads();
+    if (sl == 0) {
+        for (int q = 0; q < kSplit - 1; ++q) {
+            const double* o = s_part + ((size_t)q * 64 + lane) * 23;
+#pragma unroll
+            for (int i = 0; i < 20; ++i) L[i] += o[i];
+            C[0] += o[20];
+            C[1] += o[21];
+            C[2] += o[22];
+        }
+        finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
+    }
+}
+
+// Mapping-5 kernel: one workgroup of kSplit waves per 2x2x2 coarse block.
+template <bool RHO>
+__global__ __launch_bounds__(64 * kSplit) void level_kernel_split(LevelArgs a) {
+    __shared__ int s_nbr[27];
+    __shared__ int s_kind[27];
+    __shared__ double s_part[(kSplit - 1) * 64 * 23];
+    const int v = xcd_item(blockIdx.x, gridDim.x);
+    const int item = v >> 3, w = v & 7;
+    const int2 it = a.items[a.item0 + item];
+    if (threadIdx.x < 27) {
+        const int n = a.nbr[(size_t)it.x * 27 + threadIdx.x];
+        s_nbr[threadIdx.x] = n;
+        s_kind[threadIdx.x] = n >= 0 ? a.kind[n] : -1;
+    }
+    __syncthreads();
+    if (it.y & kItemLeaf) return;
+    m2l_item_split<RHO>(a, it.x, it.y, s_nbr, s_kind, w, (int)(threadIdx.x >> 6), s_part);
+}
+
 // Waits for this wave's LDS writes (lgkmcnt(0), global loads left in flight)
 // and keeps the compiler from moving LDS accesses across the point.  The
 // lanes of a wave execute each LDS instruction together and in order, so the
@@ -908,6 +1000,9 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
     case 304: hipLaunchKernelGGL((level_kernel_direct<RHO, 4, 3>), grid, block, 0, st, la); break;
     case 302: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 3>), grid, block, 0, st, la); break;
     case 404: hipLaunchKernelGGL((level_kernel_direct<RHO, 4, 4>), grid, block, 0, st, la); break;
+    case 504:
+        hipLaunchKernelGGL((level_kernel_split<RHO>), dim3(nitems * 8), dim3(64 * kSplit), 0, st, la);
+        break;
     case 402: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 4>), grid, block, 0, st, la); break;
     case 208: hipLaunchKernelGGL((level_kernel<RHO, 8, 2>), grid, block, 0, st, la); break;
     case 204: hipLaunchKernelGGL((level_kernel<RHO, 4, 2>), grid, block, 0, st, la); break;
@@ -1612,6 +1707,9 @@ static void plan_free(octofmm_plan* p) {
     delete p;
 }
 
+// stage_mask bit 0: stage the partner records; bit 1: the root kernel, or the
+// level kernel over the multipole-target items; bit 2: the leaf kernel over
+// the monopole-target items.  Bits 1 and 2 write disjoint sub-grids.
 static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mask) {
     octofmm_context* c = p->ctx;
     if (stage_mask & 1) {
@@ -1653,7 +1751,7 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         rr.overwrite = p->l2l ? 1 : 0;
         hipLaunchKernelGGL(root_reduce, dim3((kCells * 23 + 255) / 256), dim3(256), 0, st, rr);
         HIPC(c, hipGetLastError());
-    } else if ((stage_mask & 2) && p->nitems > 0) {
+    } else if (!p->root && (stage_mask & 6) && p->nitems > 0) {
         LevelArgs la;
         la.rec = p->d_rec;
         la.com = p->d_com;
@@ -1680,7 +1778,7 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         la.inv_dx = 1.0 / p->dx;
         la.neg_inv_dx = -1.0 / p->dx;
         const bool rho = p->type == OCTOFMM_RHO;
-        if (p->n_m2l > 0) {
+        if ((stage_mask & 2) && p->n_m2l > 0) {
             la.nitems = p->n_m2l;
             la.item0 = 0;
             if (rho)
@@ -1689,7 +1787,7 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
                 launch_level<false>(p->variant, p->n_m2l, st, la, p->precision == OCTOFMM_PRECISION_MIXED);
             HIPC(c, hipGetLastError());
         }
-        if (p->nitems > p->n_m2l) {
+        if ((stage_mask & 4) && p->nitems > p->n_m2l) {
             la.nitems = p->nitems - p->n_m2l;
             la.item0 = p->n_m2l;
             if (rho)
@@ -1702,8 +1800,9 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
     return OCTOFMM_OK;
 }
 
-// Times each workgroup shape on the live level and keeps the fastest.  The
-// trial passes write into private scratch outputs, never the caller's L / L_c.
+// Times each level-kernel mapping on the live level's multipole-target items
+// (the leaf kernel does not depend on it) and keeps the fastest.  The trial
+// passes write into private scratch outputs, never the caller's L / L_c.
 static octofmm_status plan_autotune_run(octofmm_plan* p);
 
 static octofmm_status plan_autotune(octofmm_plan* p) {
@@ -1975,7 +2074,7 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
                 forced_ok = true;
             }
     if (forced_ok) {
-    } else if ((lv->flags & OCTOFMM_PLAN_AUTOTUNE) && p->nitems > 0) {
+    } else if ((lv->flags & OCTOFMM_PLAN_AUTOTUNE) && p->n_m2l > 0 && !p->root) {
         // time each workgroup shape on the live level (outputs are scratch
         // here: the caller zeroes L before a real pass) and keep the fastest
         octofmm_status ts = plan_autotune(p);
@@ -1989,7 +2088,7 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
         hipStream_t cs = c->stream;
         if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return cleanup(OCTOFMM_ERR_HIP, "graph capture failed to start");
-        octofmm_status ls = plan_launch(p, cs, 3);
+        octofmm_status ls = plan_launch(p, cs, 7);
         hipGraph_t g = nullptr;
         const hipError_t ee = hipStreamEndCapture(cs, &g);
         if (ls != OCTOFMM_OK || ee != hipSuccess) {
@@ -2013,7 +2112,7 @@ octofmm_status octofmm_level_plan_execute(octofmm_plan* p, void* stream) {
         HIPC(c, hipGraphLaunch(p->gexec, st));
         return OCTOFMM_OK;
     }
-    return plan_launch(p, st, 3);
+    return plan_launch(p, st, 7);
 }
 
 octofmm_status octofmm_level_plan_stats(const octofmm_plan* p, octofmm_plan_stats* s) {
@@ -2033,7 +2132,7 @@ octofmm_status octofmm_level_plan_time_stages(octofmm_plan* p, void* stream, int
     for (int stage = 0; stage < 2; ++stage) {
         HIPC(c, hipEventRecord(e0, st));
         for (int i = 0; i < iters; ++i) {
-            octofmm_status s = plan_launch(p, st, 1 << stage);
+            octofmm_status s = plan_launch(p, st, stage == 0 ? 1 : 6);
             if (s != OCTOFMM_OK) return s;
         }
         HIPC(c, hipEventRecord(e1, st));
@@ -2084,6 +2183,13 @@ struct octofmm_tree_plan {
     octofmm_tree_stats stats;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // the interactions phase forks onto three more streams: one stages the
+    // partner records of every level, coarsest first, the other two run the
+    // leaf kernels of alternate levels (the leaf kernel of level l + 1 need not
+    // wait for that of level l); the level kernels stay on the caller's stream
+    hipStream_t side_stage = nullptr, side_leaf[2] = {nullptr, nullptr};
+    hipEvent_t ev_up = nullptr, ev_join_stage = nullptr, ev_join_leaf[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> ev_stage, ev_level;
 };
 
 namespace {
@@ -2092,6 +2198,15 @@ void tree_free(octofmm_tree_plan* t) {
     if (!t) return;
     if (t->gexec) (void)hipGraphExecDestroy(t->gexec);
     if (t->graph) (void)hipGraphDestroy(t->graph);
+    for (hipEvent_t e : {t->ev_up, t->ev_join_stage, t->ev_join_leaf[0], t->ev_join_leaf[1]})
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : t->ev_stage)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : t->ev_level)
+        if (e) (void)hipEventDestroy(e);
+    if (t->side_stage) (void)hipStreamDestroy(t->side_stage);
+    for (hipStream_t s : t->side_leaf)
+        if (s) (void)hipStreamDestroy(s);
     for (auto& l : t->lv) {
         plan_free(l.plan);
         (void)hipFree(l.d_kind);
@@ -2126,14 +2241,44 @@ octofmm_status tree_launch(octofmm_tree_plan* t, hipStream_t st, int phases) {
             hipLaunchKernelGGL(m2m_kernel, dim3((L.nnodes * kCells + 255) / 256), dim3(256), 0, st, a);
             HIPC(c, hipGetLastError());
         }
-    // interactions of every level, root first; each level's kernels also
+    // Interactions of every level, root first.  Each level's kernels also
     // write the final expansions (the downward pass is fused into them), so
-    // level l reads the finished level l - 1
-    if (phases & 2)
+    // both kernels of level l read the finished non-leaf sub-grids of level
+    // l - 1, which only the level kernel of l - 1 writes: the leaf kernel of
+    // level l waits for that kernel and for its own level's records, and
+    // overlaps the level kernels that follow.  Staging depends only on the
+    // upward pass.
+    if (phases & 2) {
+        hipStream_t ss = t->side_stage;
+        HIPC(c, hipEventRecord(t->ev_up, st));
+        HIPC(c, hipStreamWaitEvent(ss, t->ev_up, 0));
+        for (hipStream_t sl : t->side_leaf) HIPC(c, hipStreamWaitEvent(sl, t->ev_up, 0));
         for (int l = 0; l < nl; ++l) {
-            const octofmm_status s = plan_launch(t->lv[l].plan, st, 3);
+            const octofmm_status s = plan_launch(t->lv[l].plan, ss, 1);
             if (s != OCTOFMM_OK) return s;
+            HIPC(c, hipEventRecord(t->ev_stage[l], ss));
         }
+        for (int l = 0; l < nl; ++l) {
+            octofmm_plan* p = t->lv[l].plan;
+            HIPC(c, hipStreamWaitEvent(st, t->ev_stage[l], 0));
+            octofmm_status s = plan_launch(p, st, 2);
+            if (s != OCTOFMM_OK) return s;
+            HIPC(c, hipEventRecord(t->ev_level[l], st));
+            if (!p->root && p->nitems > p->n_m2l) {
+                hipStream_t sl = t->side_leaf[l & 1];
+                HIPC(c, hipStreamWaitEvent(sl, t->ev_stage[l], 0));
+                if (l > 0) HIPC(c, hipStreamWaitEvent(sl, t->ev_level[l - 1], 0));
+                s = plan_launch(p, sl, 4);
+                if (s != OCTOFMM_OK) return s;
+            }
+        }
+        HIPC(c, hipEventRecord(t->ev_join_stage, ss));
+        HIPC(c, hipStreamWaitEvent(st, t->ev_join_stage, 0));
+        for (int k = 0; k < 2; ++k) {
+            HIPC(c, hipEventRecord(t->ev_join_leaf[k], t->side_leaf[k]));
+            HIPC(c, hipStreamWaitEvent(st, t->ev_join_leaf[k], 0));
+        }
+    }
     return OCTOFMM_OK;
 }
 
@@ -2196,10 +2341,23 @@ octofmm_status octofmm_tree_plan_create(octofmm_context* c, const octofmm_tree* 
     std::memset(&t->stats, 0, sizeof(t->stats));
     t->stats.nlevels = nl;
     t->lv.resize(nl);
+    t->ev_stage.assign(nl, nullptr);
+    t->ev_level.assign(nl, nullptr);
     auto cleanup = [&](octofmm_status s, const char* what) {
         tree_free(t);
         return fail(c, s, "%s", what);
     };
+    {
+        bool ok = hipStreamCreateWithFlags(&t->side_stage, hipStreamNonBlocking) == hipSuccess &&
+                  hipStreamCreateWithFlags(&t->side_leaf[0], hipStreamNonBlocking) == hipSuccess &&
+                  hipStreamCreateWithFlags(&t->side_leaf[1], hipStreamNonBlocking) == hipSuccess;
+        for (hipEvent_t* e : {&t->ev_up, &t->ev_join_stage, &t->ev_join_leaf[0], &t->ev_join_leaf[1]})
+            ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
+        for (int l = 0; l < nl; ++l)
+            ok = ok && hipEventCreateWithFlags(&t->ev_stage[l], hipEventDisableTiming) == hipSuccess &&
+                 hipEventCreateWithFlags(&t->ev_level[l], hipEventDisableTiming) == hipSuccess;
+        if (!ok) return cleanup(OCTOFMM_ERR_HIP, "tree stream or event creation failed");
+    }
     for (int l = 0; l < nl; ++l) {
         const octofmm_tree_level& L = tr->levels[l];
         auto& D = t->lv[l];

@@ -1,10 +1,11 @@
 """Run the whole-tree gravity solve on one GPU and print its timings as JSON.
 
-    python -m octotiger_amd.solve [--tree c5|small] [--steps N] [--precision fp64|mixed]
+    python -m octotiger_amd.solve [--tree c5|small] [--steps N] [--precision fp64|mixed] [--no-graph]
 
 Builds the tree on the host (octotiger_amd.workloads), uploads the leaf masses,
-creates the tree plan (one hipGraph per solve, autotuned level kernels) and
-replays it N times between HIP events; then times its two phases separately:
+creates the tree plan (one hipGraph per solve, autotuned level kernels; with
+--no-graph the kernels are launched on each execute instead) and runs it N
+times between HIP events; then times its two phases separately:
 the upward M2M pass, and the interactions of every level with the downward
 L2L pass and the leaf force assembly fused into their kernels.
 """
@@ -20,6 +21,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args(argv)
 
     from . import _abi, hip, native, workloads
@@ -30,7 +32,7 @@ def main(argv=None):
     tree = workloads.config5() if args.tree == "c5" else workloads.small_tree()
     ctx = native.Context(theta=0.5, device=0)
     dev = hip.TreeOnDevice(tree)
-    flags = _abi.PLAN_USE_GRAPH | (0 if args.no_autotune else _abi.PLAN_AUTOTUNE)
+    flags = (0 if args.no_graph else _abi.PLAN_USE_GRAPH) | (0 if args.no_autotune else _abi.PLAN_AUTOTUNE)
     prec = _abi.PRECISION_MIXED if args.precision == "mixed" else _abi.PRECISION_FP64
     plan = dev.plan(ctx, precision=prec, flags=flags)
     s = hip.Stream()
@@ -48,7 +50,7 @@ def main(argv=None):
     st = plan.stats()
     pairs = sum(st[k] for k in ("far_pairs", "near_pairs", "p2p_pairs", "p2m_pairs", "root_pairs"))
     print(json.dumps({"tree": tree.name, "nodes_per_level": tree.info["nodes_per_level"],
-                      "precision": args.precision, "ms_per_solve": ms,
+                      "precision": args.precision, "graph": not args.no_graph, "ms_per_solve": ms,
                       "phases_ms": {"upward_m2m": phases[0], "interactions_l2l_forces": phases[1]},
                       "cell_interactions_per_s": pairs / (ms * 1e-3), "stats": st}), flush=True)
     plan.close()
