@@ -852,12 +852,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
     const double* wp = s_win + ((tx + h) * W + (ty + h)) * W + (tz + h) - 2048;
     double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
     const int32_t* __restrict__ list = tab->p2plist[k];
-    const double(*__restrict__ fl)[4] = tab->p2pfour[k];
+    const char* __restrict__ fc = reinterpret_cast<const char*>(tab->p2pconst);
     const int n = tab->np2p_k[k];
 #pragma unroll 8
     for (int q = 0; q < n; ++q) {
         const int e = list[q];
-        const double* f = fl[q];
+        const double* f = reinterpret_cast<const double*>(fc + (e & 0xffff));
         const double m = wp[e >> 16];
         p0 = fma(f[0], m, p0);
         p1 = fma(f[1], m, p1);

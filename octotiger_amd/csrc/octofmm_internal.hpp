@@ -49,15 +49,19 @@ struct Tables {
     uint16_t farlist[8][kMaxDelta * 8];
     uint16_t nearlist[8][kMaxDelta * 8];
     // p2p of a leaf target of class k: every stencil partner (far and near),
-    // packed as box_index | (window_offset + 2048) << 16, where the window is
-    // the sub-grid plus a halo of p2p_halo cells (edge 8 + 2 p2p_halo, x
-    // slowest) and window_offset = (dx * W + dy) * W + dz
+    // packed as const_byte_offset | (window_offset + 2048) << 16, where the
+    // window is the sub-grid plus a halo of p2p_halo cells (edge 8 + 2
+    // p2p_halo, x slowest), window_offset = (dx * W + dy) * W + dz, and
+    // const_byte_offset locates the pair's four constants in p2pconst
     int32_t p2p_halo;
     int32_t np2p_k[8];
     int32_t p2plist[8][kMaxDelta * 8];
-    // the four constants of p2plist[k][q], in the same order, so the kernel
-    // loads both without one load waiting for the other
-    double p2pfour[8][kMaxDelta * 8][4];
+    // the distinct four-constant rows the eight lists use (342 at theta 0.5),
+    // in order of first use: one compact table that every class shares, so
+    // the scalar cache holds it whole while the eight class waves stream
+    // their lists (per-class copies of the rows are five times larger)
+    int32_t np2pconst;
+    double p2pconst[kStencilBox][4];
 };
 
 // Level-kernel variants: lane mapping x 100 + waves per workgroup.

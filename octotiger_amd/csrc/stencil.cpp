@@ -151,6 +151,8 @@ bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
     const int h = 2 * maxd + 1;
     const int W = kInx + 2 * h;
     t.p2p_halo = h;
+    std::vector<int> slot(kStencilBox, -1);
+    t.np2pconst = 0;
     for (int k = 0; k < 8; ++k) {
         int q = 0;
         for (int i = 0; i < n; ++i)
@@ -161,8 +163,11 @@ bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
                 const int dz = 2 * t.delta[i][2] + (j & 1) - (k & 1);
                 const int woff = (dx * W + dy) * W + dz;
                 const int bi = box_index(dx, dy, dz);
-                for (int c = 0; c < 4; ++c) t.p2pfour[k][q][c] = t.four[bi][c];
-                t.p2plist[k][q++] = bi | ((woff + 2048) << 16);
+                if (slot[bi] < 0) {
+                    slot[bi] = t.np2pconst++;
+                    for (int c = 0; c < 4; ++c) t.p2pconst[slot[bi]][c] = t.four[bi][c];
+                }
+                t.p2plist[k][q++] = (slot[bi] * 32) | ((woff + 2048) << 16);
             }
         t.np2p_k[k] = q;
     }
