@@ -153,6 +153,22 @@ def run_tree(ctx, args, hip, _abi, np):
                      "stats": plan.stats(), "g": g}
         plan.close()
     dev.free()
+    # Octo-Tiger's own end-to-end regression value: the solid-sphere run
+    # (test_problems/sphere/CMakeLists.txt:18) prints "pot 4.470774e-01 3.176879e+00"
+    ts = workloads.solid_sphere_tree()
+    dsp = hip.TreeOnDevice(ts)
+    plan = dsp.plan(ctx, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
+    ms_sphere = time_plan(plan, stream, max(5, args.steps // 5), max(1, args.warmup), hip)
+    plan.execute(stream.handle)
+    stream.synchronize()
+    l1, l2 = workloads.sphere_pot_norms(ts, [d["g"].numpy()[..., 0] for d in dsp.levels])
+    plan.close()
+    dsp.free()
+    got = f"pot {l1:e} {l2:e}"
+    want = "pot {:e} {:e}".format(*workloads.SPHERE_POT_NORMS)
+    sphere = {"workload": f"solid sphere, max_level 3, {ts.nnodes} sub-grids, nodes per level "
+                          f"{ts.info['nodes_per_level']}", "ms_per_solve": ms_sphere,
+              "pot_norms": got, "reference": want, "matches_reference": got == want}
     st = res["fp64"]["stats"]
     pairs = st["far_pairs"] + st["near_pairs"] + st["p2p_pairs"] + st["p2m_pairs"] + st["root_pairs"]
     ga, gb = res["fp64"]["g"], res["mixed"]["g"]
@@ -171,6 +187,7 @@ def run_tree(ctx, args, hip, _abi, np):
                   "leaf_force_rms_rel_err_vs_fp64": ef, "leaf_phi_max_rel_err_vs_fp64": ephi},
         "pairs": {k: int(st[k]) for k in ("far_pairs", "near_pairs", "p2p_pairs", "p2m_pairs", "root_pairs")},
         "nodes": int(st["nodes"]), "leaves": int(st["leaves"]),
+        "solid_sphere_regression": sphere,
     }
 
 

@@ -1,13 +1,15 @@
 """Run the whole-tree gravity solve on one GPU and print its timings as JSON.
 
-    python -m octotiger_amd.solve [--tree c5|small] [--steps N] [--precision fp64|mixed] [--no-graph]
+    python -m octotiger_amd.solve [--tree c5|small|sphere] [--steps N] [--precision fp64|mixed] [--no-graph]
 
 Builds the tree on the host (octotiger_amd.workloads), uploads the leaf masses,
 creates the tree plan (one hipGraph per solve, autotuned level kernels; with
 --no-graph the kernels are launched on each execute instead) and runs it N
 times between HIP events; then times its two phases separately:
 the upward M2M pass, and the interactions of every level with the downward
-L2L pass and the leaf force assembly fused into their kernels.
+L2L pass and the leaf force assembly fused into their kernels.  For the
+solid-sphere tree it also prints the 'pot' norms Octo-Tiger's regression
+test checks (workloads.sphere_pot_norms).
 """
 import argparse
 import json
@@ -16,7 +18,7 @@ import sys
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--tree", choices=("c5", "small"), default="c5")
+    ap.add_argument("--tree", choices=("c5", "small", "sphere"), default="c5")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
@@ -29,7 +31,7 @@ def main(argv=None):
     if hip.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         return 3
-    tree = workloads.config5() if args.tree == "c5" else workloads.small_tree()
+    tree = {"c5": workloads.config5, "small": workloads.small_tree, "sphere": workloads.solid_sphere_tree}[args.tree]()
     ctx = native.Context(theta=0.5, device=0)
     dev = hip.TreeOnDevice(tree)
     flags = (0 if args.no_graph else _abi.PLAN_USE_GRAPH) | (0 if args.no_autotune else _abi.PLAN_AUTOTUNE)
@@ -48,11 +50,15 @@ def main(argv=None):
     ms = e0.elapsed_ms(e1) / args.steps
     phases = plan.time_phases(s.handle, iters=3)
     st = plan.stats()
+    extra = {}
+    if args.tree == "sphere":
+        l1, l2 = workloads.sphere_pot_norms(tree, [d["g"][..., 0] for d in dev.fetch("g")])
+        extra["pot_norms"] = f"pot {l1:e} {l2:e}"
     pairs = sum(st[k] for k in ("far_pairs", "near_pairs", "p2p_pairs", "p2m_pairs", "root_pairs"))
     print(json.dumps({"tree": tree.name, "nodes_per_level": tree.info["nodes_per_level"],
                       "precision": args.precision, "graph": not args.no_graph, "ms_per_solve": ms,
                       "phases_ms": {"upward_m2m": phases[0], "interactions_l2l_forces": phases[1]},
-                      "cell_interactions_per_s": pairs / (ms * 1e-3), "stats": st}), flush=True)
+                      "cell_interactions_per_s": pairs / (ms * 1e-3), "stats": st, **extra}), flush=True)
     plan.close()
     dev.free()
     ctx.close()

@@ -45,6 +45,7 @@ class TreeLevel:
     octant: np.ndarray      # [n] int32
     xbase: np.ndarray       # [n, 3] centre of cell (0,0,0)
     mon: np.ndarray = None  # [n, 512] cell masses of leaf nodes (zeros elsewhere)
+    rho: np.ndarray = None  # [n, 512] cell densities of leaf nodes, where a workload defines them
 
     @property
     def nnodes(self):
@@ -92,8 +93,7 @@ def build_tree(refine, max_level, lo=-0.5, size=1.0, name="tree"):
         xb = lo + cs * (INX * dxl) + 0.5 * dxl
         want = np.asarray(refine(lev, cs, xb, dxl), dtype=bool) if len(cs) else np.zeros(0, bool)
         refined[lev] = {tuple(int(v) for v in c) for c, w in zip(cs, want) if w}
-        exist.append({(2 * c[0] + (o & 1), 2 * c[1] + ((o >> 1) & 1), 2 * c[2] + ((o >> 2) & 1))
-                      for c in refined[lev] for o in range(8)})
+        exist.append(_children(refined[lev]))
     # nesting, bottom-up: neighbours of refined nodes must exist
     for lev in range(max_level - 1, 0, -1):
         n_side = 2 ** lev
@@ -112,11 +112,63 @@ def build_tree(refine, max_level, lo=-0.5, size=1.0, name="tree"):
     exist = [{(0, 0, 0)}]
     for lev in range(max_level):
         refined[lev] &= exist[lev]
-        exist.append({(2 * c[0] + (o & 1), 2 * c[1] + ((o >> 1) & 1), 2 * c[2] + ((o >> 2) & 1))
-                      for c in refined[lev] for o in range(8)})
+        exist.append(_children(refined[lev]))
+    return _assemble(exist, refined, lo, size, name)
+
+
+def _children(nodes):
+    return {(2 * c[0] + (o & 1), 2 * c[1] + ((o >> 1) & 1), 2 * c[2] + ((o >> 2) & 1)) for c in nodes for o in range(8)}
+
+
+def regrid_tree(flag, max_level, passes, lo=-1.0, size=2.0, name="tree"):
+    """The tree Octo-Tiger's start-up regrid passes leave behind.
+
+    Each pass (node_server::regrid, src/node_server_actions_1.cpp:244-279)
+    asks every existing node whether it wants refinement
+    (check_for_refinement, src/node_server_actions_2.cpp:36-73):
+    flag(level, coords[n,3]) -> bool[n].  A flagged node forces its 26
+    in-domain same-level neighbours to exist, i.e. flags their parents, and
+    so on up the tree (force_nodes_to_exist, :387-451).  The flagged set is
+    the new refinement: refined nodes without a flag lose their children,
+    unrefined flagged nodes get 8 new leaf children (regrid_gather,
+    src/node_server_actions_1.cpp:68-124), which the next pass examines.
+    The frontend runs max_level + 1 such passes before the first solve
+    (frontend/frontend-helper.cpp:190-200).
+    """
+    refined = [set() for _ in range(max_level + 1)]
+    for _ in range(passes):
+        exist = [{(0, 0, 0)}]
+        for lev in range(max_level):
+            exist.append(_children(refined[lev]))
+        flagged = [set() for _ in range(max_level + 1)]
+        for lev in range(max_level + 1):
+            cs = sorted(exist[lev])
+            if cs:
+                want = np.asarray(flag(lev, np.array(cs, dtype=np.int64)), dtype=bool)
+                flagged[lev] = {c for c, w in zip(cs, want) if w}
+        for lev in range(max_level, 0, -1):
+            n_side = 1 << lev
+            for c in flagged[lev]:
+                for dz in (-1, 0, 1):
+                    for dy in (-1, 0, 1):
+                        for dx_ in (-1, 0, 1):
+                            q = (c[0] + dx_, c[1] + dy, c[2] + dz)
+                            if min(q) < 0 or max(q) >= n_side:
+                                continue
+                            p = (q[0] >> 1, q[1] >> 1, q[2] >> 1)
+                            if p not in exist[lev - 1]:
+                                raise ValueError("regrid pass left a refined node without neighbours")
+                            flagged[lev - 1].add(p)
+        refined = flagged
+    exist = [{(0, 0, 0)}]
+    for lev in range(max_level):
+        exist.append(_children(refined[lev]))
+    return _assemble(exist, refined, lo, size, name)
+
+
+def _assemble(exist, refined, lo, size, name):
     while len(exist) > 1 and not exist[-1]:
         exist.pop()
-
     levels = []
     prev_index = None
     for lev, nodes in enumerate(exist):

@@ -397,3 +397,121 @@ def small_tree(seed=SEED_BASE + 50, max_level=2, name="small-tree"):
     t.info = {"seed": seed, "nodes_per_level": [lv.nnodes for lv in t.levels],
               "leaves_per_level": [lv.nleaves for lv in t.levels], "nodes": t.nnodes}
     return t
+
+
+# ---------------------------------------------------------------------------
+# Octo-Tiger's solid-sphere regression problem (test_problems/sphere):
+# problem=solid_sphere, max_level=3, hydro=off, stop_step=1
+# ---------------------------------------------------------------------------
+SPHERE_CENTRE = (0.25, 0.0, 0.0)     # solid_sphere_{x,y,z}center defaults (src/options_processing.cpp:80-82)
+SPHERE_RADIUS = 1.0 / 3.0            # solid_sphere_radius (:83)
+SPHERE_MASS = 1.0                    # solid_sphere_mass (:84)
+SPHERE_RHO_MIN = 1.0e-12             # solid_sphere_rho_min (:85)
+SPHERE_SUBSAMPLES = 25               # N of solid_sphere (src/problem.cpp:404)
+SPHERE_REFINE_R2 = 1.0 / 3.0         # ssr0 of refine_test_center (src/problem.cpp:30)
+SPHERE_POT_NORMS = (4.470774e-01, 3.176879e+00)   # test_problems/sphere/CMakeLists.txt:18 (INX = 8)
+INX_SPHERE = 8
+
+
+def solid_sphere_density(X, dx):
+    """Cell densities of solid_sphere (src/problem.cpp:403-452) for cell
+    centres X[..., 3] of edge dx, with the same floating-point steps: cells
+    wholly inside get M / V, cells the surface cuts count their N^3
+    sub-cell centres strictly inside (adding drho once per centre), every
+    cell is floored at rho_min."""
+    X = np.asarray(X, dtype=np.float64)
+    shape = X.shape[:-1]
+    X = X.reshape(-1, 3)
+    N = SPHERE_SUBSAMPLES
+    r0 = SPHERE_RADIUS
+    V = 4.0 / 3.0 * np.pi * r0 * r0 * r0
+    drho = SPHERE_MASS / float(N * N * N) / V
+    acc = np.concatenate([[0.0], np.cumsum(np.full(N * N * N, drho))])   # drho added k times, in order
+    full = drho * N * N * N
+    h = dx / 2.0
+    rel = X - np.array(SPHERE_CENTRE)
+    lo, hi = rel - h, rel + h   # x0 - dx/2, x0 + dx/2
+    vmax = np.maximum(np.abs(hi), np.abs(lo))
+    vmin = np.where(hi * lo < 0.0, 0.0, np.where(hi > 0.0, np.minimum(hi, lo), np.maximum(hi, lo)))
+    r2 = r0 * r0
+    inside = (vmax[:, 0] * vmax[:, 0] + vmax[:, 1] * vmax[:, 1] + vmax[:, 2] * vmax[:, 2]) <= r2
+    cut = ~inside & ((vmin[:, 0] * vmin[:, 0] + vmin[:, 1] * vmin[:, 1] + vmin[:, 2] * vmin[:, 2]) <= r2)
+    rho = np.zeros(len(X))
+    rho[inside] = full
+    d = dx / N
+    k = np.arange(N, dtype=np.float64)
+    for c in np.nonzero(cut)[0]:
+        o = (rel[c] - h) + d / 2.0
+        sx = (o[0] + k * d) ** 2
+        sy = (o[1] + k * d) ** 2
+        sz = (o[2] + k * d) ** 2
+        cnt = int(((sx[:, None, None] + sy[None, :, None]) + sz[None, None, :] < r2).sum())
+        rho[c] = acc[cnt]
+    return np.maximum(rho, SPHERE_RHO_MIN).reshape(shape)
+
+
+def solid_sphere_tree(max_level=3, passes=None, name="solid-sphere"):
+    """The octree and leaf masses of Octo-Tiger's solid-sphere regression
+    run.  Domain [-1, 1]^3 (xscale = 1), 8^3 cells per node.  Refinement is
+    refine_test_center (src/problem.cpp:227-233): a node below max_level
+    refines when a cell it scans lies within r^2 < 1/3 of the origin, the
+    root always (min_level = 1, grid::refine_me, src/grid.cpp:1493-1548,
+    which scans the interior plus a REFINE_BW = 2 face band, skipping band
+    cells on edges and corners).  The tree is what max_level + 1 regrid
+    passes leave (tree.regrid_tree; passes overrides the count).  Leaf cells carry the solid_sphere
+    density at their centres and mass rho dx^3 (grid::compute_multipoles,
+    src/grid_fmm.cpp:1643, 1774)."""
+    from . import tree as _tree
+
+    rel = np.arange(-2, INX_SPHERE + 2)
+    R = np.stack(np.meshgrid(rel, rel, rel, indexing="ij"), axis=-1).reshape(-1, 3)
+    R = R[((R < 0) | (R >= INX_SPHERE)).sum(axis=1) <= 1].astype(np.float64)
+
+    def flag(lev, cs):
+        if lev < 1:
+            return np.ones(len(cs), bool)
+        if lev >= max_level:
+            return np.zeros(len(cs), bool)
+        dx = 2.0 / float(INX_SPHERE << lev)
+        xmin = cs.astype(np.float64) * (2.0 / float(1 << lev)) - 1.0
+        X = (R[None, :, :] + 0.5) * dx + xmin[:, None, :]
+        r2 = X[..., 0] * X[..., 0] + X[..., 1] * X[..., 1] + X[..., 2] * X[..., 2]
+        return (r2 < SPHERE_REFINE_R2).any(axis=1)
+
+    npass = max_level + 1 if passes is None else passes
+    t = _tree.regrid_tree(flag, max_level, npass, lo=-1.0, size=2.0, name=name)
+    for lv in t.levels:
+        leaf = lv.kind == _abi.SLOT_LEAF
+        rho = np.zeros((lv.nnodes, 512))
+        if leaf.any():
+            i = np.arange(INX_SPHERE, dtype=np.float64)
+            g = np.stack(np.meshgrid(i, i, i, indexing="ij"), axis=-1).reshape(512, 3)
+            xmin = lv.coords[leaf].astype(np.float64) * (2.0 / float(1 << lv.level)) - 1.0
+            rho[leaf] = solid_sphere_density((g[None] + 0.5) * lv.dx + xmin[:, None, :], lv.dx)
+        lv.rho = np.ascontiguousarray(rho)
+        lv.mon = np.ascontiguousarray(rho * (lv.dx * lv.dx * lv.dx))
+    t.info = {"problem": "solid_sphere", "max_level": max_level,
+              "nodes_per_level": [lv.nnodes for lv in t.levels],
+              "leaves_per_level": [lv.nleaves for lv in t.levels], "nodes": t.nnodes}
+    return t
+
+
+def sphere_pot_norms(tree, phi_levels):
+    """The 'pot' line of node_server::compare_analytic
+    (src/node_server_actions_2.cpp:154-182): the analytic pot field of the
+    solid sphere is zero (solid_sphere never sets it), so grid::compute_analytic
+    (src/grid.cpp:1796-1800) sums |phi rho| dV and (phi rho)^2 dV over the
+    interior cells of the leaves, with pot = phi rho (src/grid_fmm.cpp:1628);
+    the root divides by the domain volume (2 xscale)^3 = 8 and takes the root
+    of the L2 sum.  phi_levels[l] is [n, 512] (leaf rows are used)."""
+    l1 = l2 = 0.0
+    for lv, phi in zip(tree.levels, phi_levels):
+        leaf = lv.kind == _abi.SLOT_LEAF
+        if not leaf.any():
+            continue
+        dv = lv.dx * lv.dx * lv.dx
+        pot = np.asarray(phi)[leaf] * lv.rho[leaf]
+        l1 += float(np.abs(pot).sum()) * dv
+        l2 += float((pot * pot).sum()) * dv
+    vol = 8.0
+    return l1 / vol, float(np.sqrt(l2 / vol))

@@ -161,3 +161,15 @@ def test_config5_mixed_precision_force_error(ctx):
     assert ef <= 1e-6 and ephi <= 1e-6
     d64.free()
     d32.free()
+
+
+def test_solid_sphere_reproduces_reference_regression(ctx):
+    """Octo-Tiger's solid-sphere regression (test_problems/sphere, max_level 3):
+    the GPU solve matches the oracle and prints the reference's pot norms."""
+    t = workloads.solid_sphere_tree()
+    print(f"tree {t.name} nodes {t.info['nodes_per_level']}", flush=True)
+    _, dev, got = run(ctx, t)
+    compare(t, got, oracle.tree_solve(t))
+    l1, l2 = workloads.sphere_pot_norms(t, [r["g"][..., 0] for r in got])
+    assert f"pot {l1:e} {l2:e}" == "pot 4.470774e-01 3.176879e+00"
+    dev.free()

@@ -126,3 +126,52 @@ def test_drhodt_tree_solve_is_linear_in_mass_rates():
     b = oracle.tree_solve(t2, type_=_abi.DRHODT, coms=prior)
     for x, y in zip(a, b):
         np.testing.assert_allclose(y["L"], 2.0 * x["L"], rtol=1e-12, atol=1e-12 * np.abs(x["L"]).max())
+
+
+def _regex_form(l1, l2):
+    # compare_analytic prints "%16s %e %e %e" (src/node_server_actions_2.cpp:179);
+    # the regression regex is "pot 4.470774e-01 3.176879e.00"
+    return f"pot {l1:e} {l2:e}"
+
+
+def test_solid_sphere_tree_is_the_regrid_fixed_point():
+    t = workloads.solid_sphere_tree()
+    octree.check_tree(t)
+    assert t.info["nodes_per_level"] == [1, 8, 64, 256]
+    assert t.info["leaves_per_level"] == [0, 0, 32, 256]
+    # one more regrid pass changes nothing; one fewer has not reached level 3
+    again = workloads.solid_sphere_tree(passes=5)
+    assert workloads.solid_sphere_tree(passes=3).info["nodes_per_level"] == [1, 8, 64, 256]
+    assert workloads.solid_sphere_tree(passes=2).info["nodes_per_level"] == [1, 8, 64]
+    for a, b in zip(t.levels, again.levels):
+        assert np.array_equal(a.coords, b.coords) and np.array_equal(a.kind, b.kind)
+    # the sphere's mass is 1 up to the N^3 sub-sampling of the cut cells
+    assert abs(sum(lv.mon.sum() for lv in t.levels) - 1.0) < 2e-5
+
+
+def test_solid_sphere_density_matches_direct_subsampling():
+    # straight loops of src/problem.cpp:403-452 for cells the surface cuts
+    rng = np.random.default_rng(5)
+    r0, N = workloads.SPHERE_RADIUS, workloads.SPHERE_SUBSAMPLES
+    drho = 1.0 / float(N * N * N) / (4.0 / 3.0 * np.pi * r0 * r0 * r0)
+    dx = 2.0 / 64
+    for _ in range(6):
+        d = rng.normal(size=3)
+        c = np.array(workloads.SPHERE_CENTRE) + d / np.linalg.norm(d) * r0 + rng.uniform(-0.3, 0.3, 3) * dx
+        x0, y0, z0 = (c - np.array(workloads.SPHERE_CENTRE)) - dx / 2.0 + dx / N / 2.0
+        u = 0.0
+        for i in range(N):
+            for j in range(N):
+                for k in range(N):
+                    if (x0 + i * dx / N) ** 2 + (y0 + j * dx / N) ** 2 + (z0 + k * dx / N) ** 2 < r0 * r0:
+                        u += drho
+        assert workloads.solid_sphere_density(c, dx) == max(u, 1e-12)
+
+
+def test_solid_sphere_pot_norms_match_reference_regression():
+    """Octo-Tiger's own end-to-end regression value for this problem
+    (test_problems/sphere/CMakeLists.txt:18, INX = 8, theta = 0.5)."""
+    t = workloads.solid_sphere_tree()
+    out = oracle.tree_solve(t)
+    l1, l2 = workloads.sphere_pot_norms(t, [o["g"][..., 0] for o in out])
+    assert _regex_form(l1, l2) == "pot 4.470774e-01 3.176879e+00"
