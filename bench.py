@@ -300,6 +300,11 @@ def main():
                  "valu_frac": s3["flops"] / (c3["fp64"]["stage_ms"][1] * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
                  "kernel_variant": int(s3["variant"])}
 
+    # dense vector FMA rates of this device: FP64, and FP32 as all device code
+    # here is built (packed FP32 off), the rate the mixed-precision pairs get
+    fma_rate = {"fp64": ctx.measure_fma_rate(_abi.PRECISION_FP64),
+                "fp32_unpacked": ctx.measure_fma_rate(_abi.PRECISION_MIXED)}
+
     tree = None
     if not args.no_c5:
         tree = run_tree(ctx, args, hip, _abi, np)
@@ -337,9 +342,13 @@ def main():
             "traffic": traffic,
             "kernel_ms": k_ms,
             "staging_ms": c2["fp64"]["stage_ms"][0],
+            # the dense FP64 FMA rate of this box, re-measured (octofmm_measure_fma_rate)
+            "measured_peak": fma_rate["fp64"],
+            "frac_of_measured_peak": achieved / fma_rate["fp64"],
         },
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "fma_rate_tflops": fma_rate,
         "mixed_kind_level": kinds,
         "tree_solve": tree,
         "parity_check": check,

@@ -296,6 +296,18 @@ octofmm_status octofmm_tree_plan_time_phases(octofmm_tree_plan* plan, void* stre
     double* ms);
 octofmm_status octofmm_tree_plan_destroy(octofmm_tree_plan* plan);
 
+/* ---------------------------------------------------------------------------
+ * Device FMA rate (diagnostic).  No counterpart in the reference: it measures
+ * the dense vector FMA rate of the context's device, the peak the interaction
+ * kernels are judged against.  Every lane runs 8 independent FMA chains
+ * (precision OCTOFMM_PRECISION_FP64: v_fma_f64; OCTOFMM_PRECISION_MIXED: the
+ * FP32 form, unpacked as all device code here is built) over enough
+ * workgroups to fill every CU; *tflops = 2 x FMAs / time of the timed launch,
+ * run on the context's stream after one warm-up.  Allocates and frees an
+ * 8 MiB scratch output; synchronous.
+ * ------------------------------------------------------------------------- */
+octofmm_status octofmm_measure_fma_rate(octofmm_context* ctx, int32_t precision, double* tflops);
+
 #ifdef __cplusplus
 }
 #endif

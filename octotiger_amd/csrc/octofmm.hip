@@ -1057,6 +1057,25 @@ __global__ __launch_bounds__(256) void root_reduce(RootReduceArgs a) {
 }
 
 // Fill kernel used instead of hipMemsetAsync so the passes stay graph-safe.
+// Dense FMA rate probe: 8 independent chains per lane, so the issue rate and
+// not the FMA latency bounds it.  b < 1 keeps the chains bounded, away from
+// overflow and denormals (they converge to c / (1 - b)).
+template <typename T>
+__global__ __launch_bounds__(256) void fma_rate_kernel(T* __restrict__ out, T b, T c, int iters) {
+    T a[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = (T)(threadIdx.x & 63) + (T)i;
+#pragma unroll 4
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a[i] = fma(a[i], b, c);
+    }
+    T s = a[0];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) s += a[i];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 __global__ void zero_fill(double* p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -2504,6 +2523,46 @@ octofmm_status octofmm_tree_plan_time_phases(octofmm_tree_plan* t, void* stream,
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return OCTOFMM_OK;
+}
+
+octofmm_status octofmm_measure_fma_rate(octofmm_context* c, int32_t precision, double* tflops) {
+    if (!c || !tflops) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    if (precision != OCTOFMM_PRECISION_FP64 && precision != OCTOFMM_PRECISION_MIXED)
+        return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "precision must be FP64 or MIXED (FP32)");
+    HIPC(c, hipSetDevice(c->device));
+    // 4096 workgroups of 256 lanes (16 per CU on 256 CUs), 8 chains x 2048 steps each
+    constexpr int kBlocks = 4096, kIters = 2048;
+    void* out = nullptr;
+    HIPC(c, hipMalloc(&out, sizeof(double) * (size_t)kBlocks * 256));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto run = [&]() {
+        if (precision == OCTOFMM_PRECISION_FP64)
+            hipLaunchKernelGGL(fma_rate_kernel<double>, dim3(kBlocks), dim3(256), 0, c->stream,
+                static_cast<double*>(out), 0.999999, 1.0e-7, kIters);
+        else
+            hipLaunchKernelGGL(fma_rate_kernel<float>, dim3(kBlocks), dim3(256), 0, c->stream,
+                static_cast<float*>(out), 0.999999f, 1.0e-7f, kIters);
+    };
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        run();
+        e = hipEventRecord(e0, c->stream);
+    }
+    if (e == hipSuccess) {
+        run();
+        e = hipEventRecord(e1, c->stream);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    HIPC(c, e);
+    const double fmas = (double)kBlocks * 256.0 * 8.0 * kIters;
+    *tflops = ms > 0.f ? 2.0 * fmas / (ms * 1e-3) / 1e12 : 0.0;
     return OCTOFMM_OK;
 }
 

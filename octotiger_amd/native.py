@@ -20,6 +20,7 @@ EXPORTED = [
     "octofmm_level_plan_execute", "octofmm_level_plan_stats", "octofmm_level_plan_time_stages",
     "octofmm_level_plan_destroy", "octofmm_tree_plan_create", "octofmm_tree_plan_execute",
     "octofmm_tree_plan_stats", "octofmm_tree_plan_time_phases", "octofmm_tree_plan_destroy",
+    "octofmm_measure_fma_rate",
 ]
 
 _lib = None
@@ -60,6 +61,7 @@ def lib():
         L.octofmm_tree_plan_stats.argtypes = [vp, ctypes.POINTER(_abi.TreeStats)]
         L.octofmm_tree_plan_time_phases.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
         L.octofmm_tree_plan_destroy.argtypes = [vp]
+        L.octofmm_measure_fma_rate.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
         for fn in EXPORTED:
             getattr(L, fn).restype = getattr(L, fn).restype or ctypes.c_int
         L.octofmm_status_string.restype = ctypes.c_char_p
@@ -113,6 +115,14 @@ class Context:
         (addresses as ints, e.g. torch tensor data_ptr())."""
         return LevelPlan(self, level, d_M, d_com, d_mon, d_L, d_L_c, type_, precision, mixed_far_d2,
                          flags, targets)
+
+    def measure_fma_rate(self, precision=_abi.PRECISION_FP64):
+        """Dense vector FMA rate of the device in TFLOP/s (FP64, or unpacked
+        FP32 with PRECISION_MIXED), from one timed launch of 8 independent
+        FMA chains per lane over the whole chip."""
+        out = ctypes.c_double()
+        self._check(lib().octofmm_measure_fma_rate(self._h, precision, ctypes.byref(out)))
+        return float(out.value)
 
     def tree_plan(self, tree, dev, **kw):
         """Plan of the whole gravity solve over a tree.Tree (see TreePlan)."""
