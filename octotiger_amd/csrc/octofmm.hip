@@ -574,9 +574,11 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
         return ns < 0 ? 0u : fb;
     };
     const double2* group = stage + c * kDirGroup;
-    issue(0);
-    for (int di = 0; di < nd; ++di) {
-        const unsigned fb = far_bits(di);
+    // only the offsets with a far pair are staged (delta 0 is all near / self)
+    const int nfd = tab->nfar_delta;
+    if (nfd > 0) issue(tab->far_delta[0]);
+    for (int q = 0; q < nfd; ++q) {
+        const unsigned fb = far_bits(tab->far_delta[q]);
         __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): this wave's image is in LDS
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -589,7 +591,7 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             }
         }
         lds_wave_sync();   // every read of the image done before it is refilled
-        if (di + 1 < nd) issue(di + 1);
+        if (q + 1 < nfd) issue(tab->far_delta[q + 1]);
     }
     if (flags & kItemNear) {
         for (int di = 0; di < nd; ++di) {

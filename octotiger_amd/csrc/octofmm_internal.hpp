@@ -38,6 +38,8 @@ struct Tables {
     int8_t delta[kMaxDelta][4];       // coarse offsets (x, y, z, 0)
     uint64_t farbits[kMaxDelta];      // byte k: bit j set if (delta, j, k) is far
     uint64_t nearbits[kMaxDelta];     // byte k: bit j set if (delta, j, k) is near (self excluded)
+    int32_t nfar_delta;               // coarse offsets with a far pair for some child class,
+    int32_t far_delta[kMaxDelta];     // in delta order (delta 0 has none: it is all near / self)
     double four[kStencilBox][4];      // p2p constants (-1/r, x/r^3, y/r^3, z/r^3) by fine offset
     uint8_t root_excluded[kStencilBox];   // root kernel: near offsets and self
     uint8_t phase[kStencilBox];       // 1 far, 0 near (inside the stencil), 1 elsewhere

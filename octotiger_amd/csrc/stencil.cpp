@@ -131,6 +131,9 @@ bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
             }
     t.ndelta = n;
     t.max_delta = maxd;
+    t.nfar_delta = 0;
+    for (int i = 0; i < n; ++i)
+        if (t.farbits[i]) t.far_delta[t.nfar_delta++] = i;
     int nf = 0, nn = 0;
     for (int i = 0; i < n; ++i) {
         nf += __builtin_popcountll(t.farbits[i] & 0xffull);
