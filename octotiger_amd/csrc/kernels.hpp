@@ -57,7 +57,7 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 __device__ __forceinline__ void d_tensor(double x, double y, double z, DTensor& t) {
     const double x00 = x * x;
     double r2 = fma(z, z, fma(y, y, x00));
-    r2 = r2 < 1.0e-20 ? 1.0e-20 : r2;
+    r2 = fmax(r2, 1.0e-20);   // one v_max_f64; r2 of finite separations is never NaN
     const double ri = rsqrt_nr(r2);           // 1/r
     const double r2inv = ri * ri;
     const double d1 = ri * r2inv;             // 1/r^3
