@@ -51,6 +51,8 @@ def kernel_name(variant):
     """Device kernel the plan's interaction launch runs for a variant
     (lane mapping x 100 + waves per workgroup)."""
     mapping, waves = divmod(variant, 100)
+    if mapping == 5:
+        return f"level_kernel_split<RHO, {waves}>"
     if mapping >= 3:
         return f"level_kernel_direct<RHO, {waves}, {mapping}>"
     return f"level_kernel<RHO, {waves}, {mapping}>"

@@ -301,14 +301,12 @@ __device__ void m2l_item(const LevelArgs& a, int slot, int flags, const int* s_n
 }
 
 // Mapping 5: the coarse offsets of one 2x2x2 coarse block are split over
-// kSplit waves of a workgroup (wave s takes offsets s, s + kSplit, ...), so a
-// sub-grid runs on 8 * kSplit waves; this shortens the serial time of one
+// SPLIT waves of a workgroup (wave s takes offsets s, s + SPLIT, ...), so a
+// sub-grid runs on 8 * SPLIT waves; this shortens the serial time of one
 // item for levels with few sub-grids.  Waves s > 0 leave their partial
 // expansions in LDS and wave 0 adds them in the order s = 1, 2, ... before
 // writing the result.
-constexpr int kSplit = 4;
-
-template <bool RHO>
+template <bool RHO, int SPLIT>
 __device__ void m2l_item_split(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind, int w,
     int sl, double* __restrict__ s_part) {
     const LaneGeom g = lane_geom(w);
@@ -324,7 +322,7 @@ __device__ void m2l_item_split(const LevelArgs& a, int slot, int flags, const in
 #pragma unroll
     for (int i = 0; i < 20; ++i) L[i] = 0.0;
     const int nd = tab->ndelta;
-    for (int di = sl; di < nd; di += kSplit) {
+    for (int di = sl; di < nd; di += SPLIT) {
         const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
         unsigned fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
         int ns, raw;
@@ -361,7 +359,7 @@ __device__ void m2l_item_split(const LevelArgs& a, int slot, int flags, const in
     }
     __syncthreads();
     if (sl == 0) {
-        for (int q = 0; q < kSplit - 1; ++q) {
+        for (int q = 0; q < SPLIT - 1; ++q) {
             const double* o = s_part + ((size_t)q * 64 + lane) * 23;
 #pragma unroll
             for (int i = 0; i < 20; ++i) L[i] += o[i];
@@ -373,12 +371,12 @@ __device__ void m2l_item_split(const LevelArgs& a, int slot, int flags, const in
     }
 }
 
-// Mapping-5 kernel: one workgroup of kSplit waves per 2x2x2 coarse block.
-template <bool RHO>
-__global__ __launch_bounds__(64 * kSplit) void level_kernel_split(LevelArgs a) {
+// Mapping-5 kernel: one workgroup of SPLIT waves per 2x2x2 coarse block.
+template <bool RHO, int SPLIT>
+__global__ __launch_bounds__(64 * SPLIT) void level_kernel_split(LevelArgs a) {
     __shared__ int s_nbr[27];
     __shared__ int s_kind[27];
-    __shared__ double s_part[(kSplit - 1) * 64 * 23];
+    __shared__ double s_part[(SPLIT - 1) * 64 * 23];
     const int v = xcd_item(blockIdx.x, gridDim.x);
     const int item = v >> 3, w = v & 7;
     const int2 it = a.items[a.item0 + item];
@@ -389,7 +387,7 @@ __global__ __launch_bounds__(64 * kSplit) void level_kernel_split(LevelArgs a) {
     }
     __syncthreads();
     if (it.y & kItemLeaf) return;
-    m2l_item_split<RHO>(a, it.x, it.y, s_nbr, s_kind, w, (int)(threadIdx.x >> 6), s_part);
+    m2l_item_split<RHO, SPLIT>(a, it.x, it.y, s_nbr, s_kind, w, (int)(threadIdx.x >> 6), s_part);
 }
 
 // Waits for this wave's LDS writes (lgkmcnt(0), global loads left in flight)
@@ -1003,7 +1001,7 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
     case 302: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 3>), grid, block, 0, st, la); break;
     case 404: hipLaunchKernelGGL((level_kernel_direct<RHO, 4, 4>), grid, block, 0, st, la); break;
     case 504:
-        hipLaunchKernelGGL((level_kernel_split<RHO>), dim3(nitems * 8), dim3(64 * kSplit), 0, st, la);
+        hipLaunchKernelGGL((level_kernel_split<RHO, 4>), dim3(nitems * 8), dim3(256), 0, st, la);
         break;
     case 402: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 4>), grid, block, 0, st, la); break;
     case 208: hipLaunchKernelGGL((level_kernel<RHO, 8, 2>), grid, block, 0, st, la); break;

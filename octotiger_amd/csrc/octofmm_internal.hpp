@@ -78,7 +78,8 @@ struct Tables {
 //              waves per SIMD fit the 12 KiB LDS image of each wave.
 //   mapping 4: mapping 3 with the image double buffered by child halves.
 //   mapping 5: the coarse offsets of each 2x2x2 coarse block split over 4
-//              waves (variant 504), for levels with few sub-grids.
+//              waves (variant 504), for levels with few sub-grids.  A 2-way
+//              split was measured slower at every level size tried.
 constexpr int kVariants[14] = {8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504};
 
 // Builds the tables; returns false when theta is below the floor or the
