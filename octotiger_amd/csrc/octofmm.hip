@@ -818,7 +818,7 @@ __device__ void m2l_item_classwave(const LevelArgs& a, int slot, int flags, cons
 // the 1/dx and -1/dx^2 factors are applied once per cell).  Refined
 // neighbours then contribute through p2m with their full multipoles over the
 // same stencil, far and near alike (monopole_kernel_templates.hpp:69-239).
-template <bool RHO>
+template <bool RHO, bool CUBE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void leaf_kernel(LevelArgs a) {
     extern __shared__ double s_win[];
     __shared__ int s_nbr[27];
@@ -832,8 +832,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
     }
     __syncthreads();
     const Tables* __restrict__ tab = a.tab;
-    const int h = tab->p2p_halo, W = kInx + 2 * h;
-    for (int i = threadIdx.x; i < W * W * W; i += 512) {
+    // window edge: 14 (halo 3) on the cube path, known at compile time, so the
+    // gather's index arithmetic folds and its loads are all in flight at once
+    const int h = CUBE ? 3 : tab->p2p_halo, W = kInx + 2 * h;
+    auto gather = [&](int i) {
         const int x = i / (W * W) - h, y = (i / W) % W - h, z = i % W - h;
         const int raw = (x < 0 ? 0 : x < kInx ? 1 : 2) + 3 * (y < 0 ? 0 : y < kInx ? 1 : 2) +
             9 * (z < 0 ? 0 : z < kInx ? 1 : 2);
@@ -842,27 +844,59 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
         if (ns >= 0 && s_kind[raw] == OCTOFMM_SLOT_LEAF)
             m = a.mon[(size_t)ns * kCells + ((x + kInx) & 7) * 64 + ((y + kInx) & 7) * 8 + ((z + kInx) & 7)];
         s_win[i] = m;
+    };
+    if constexpr (CUBE) {
+        constexpr int kWin = 14 * 14 * 14;
+#pragma unroll
+        for (int r = 0; r < (kWin + 511) / 512; ++r)
+            if (threadIdx.x + r * 512 < kWin) gather(threadIdx.x + r * 512);
+    } else {
+        for (int i = threadIdx.x; i < W * W * W; i += 512) gather(i);
     }
     __syncthreads();
     const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int l = threadIdx.x & 63;
     const int Cx = l >> 4, Cy = (l >> 2) & 3, Cz = l & 3;
-    const int tx = 2 * Cx + ((k >> 2) & 1), ty = 2 * Cy + ((k >> 1) & 1), tz = 2 * Cz + (k & 1);
+    const int kx = (k >> 2) & 1, ky = (k >> 1) & 1, kz = k & 1;
+    const int tx = 2 * Cx + kx, ty = 2 * Cy + ky, tz = 2 * Cz + kz;
     const int cell = tx * 64 + ty * 8 + tz;
-    const double* wp = s_win + ((tx + h) * W + (ty + h)) * W + (tz + h) - 2048;
     double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
-    const int32_t* __restrict__ list = tab->p2plist[k];
-    const char* __restrict__ fc = reinterpret_cast<const char*>(tab->p2pconst);
-    const int n = tab->np2p_k[k];
+    if constexpr (CUBE) {
+        // the 6^3 block 2C - 2 .. 2C + 3 around the target's coarse cell C
+        // (the target itself included: four of offset 0 is zero), with the
+        // window offsets as LDS immediates and the constants of offset
+        // d = p - k at fixed offsets from one class base:
+        // box(d) = box(p) + 3 * 133 - (121 kx + 11 ky + kz) for p in [0, 6)^3
+        const double* wc = s_win + ((2 * Cx + 1) * 14 + (2 * Cy + 1)) * 14 + (2 * Cz + 1);
+        const double(*__restrict__ fb)[4] = tab->four + (3 * 133 - (121 * kx + 11 * ky + kz));
+#pragma unroll
+        for (int px = 0; px < 6; ++px)
+#pragma unroll
+            for (int py = 0; py < 6; ++py)
+#pragma unroll
+                for (int pz = 0; pz < 6; ++pz) {
+                    const double m = wc[(px * 14 + py) * 14 + pz];
+                    const double* f = fb[px * 121 + py * 11 + pz];
+                    p0 = fma(f[0], m, p0);
+                    p1 = fma(f[1], m, p1);
+                    p2 = fma(f[2], m, p2);
+                    p3 = fma(f[3], m, p3);
+                }
+    } else {
+        const double* wp = s_win + ((tx + h) * W + (ty + h)) * W + (tz + h) - 2048;
+        const int32_t* __restrict__ list = tab->p2plist[k];
+        const char* __restrict__ fc = reinterpret_cast<const char*>(tab->p2pconst);
+        const int n = tab->np2p_k[k];
 #pragma unroll 8
-    for (int q = 0; q < n; ++q) {
-        const int e = list[q];
-        const double* f = reinterpret_cast<const double*>(fc + (e & 0xffff));
-        const double m = wp[e >> 16];
-        p0 = fma(f[0], m, p0);
-        p1 = fma(f[1], m, p1);
-        p2 = fma(f[2], m, p2);
-        p3 = fma(f[3], m, p3);
+        for (int q = 0; q < n; ++q) {
+            const int e = list[q];
+            const double* f = reinterpret_cast<const double*>(fc + (e & 0xffff));
+            const double m = wp[e >> 16];
+            p0 = fma(f[0], m, p0);
+            p1 = fma(f[1], m, p1);
+            p2 = fma(f[2], m, p2);
+            p3 = fma(f[3], m, p3);
+        }
     }
     const double s1 = a.inv_dx * a.neg_inv_dx;
     double L[4] = {p0 * a.inv_dx, p1 * s1, p2 * s1, p3 * s1};
@@ -917,10 +951,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
 }
 
 template <bool RHO>
-void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, int halo) {
-    const int W = kInx + 2 * halo;
+void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, const Tables& tab) {
+    const int W = kInx + 2 * tab.p2p_halo;
     const size_t lds = sizeof(double) * std::max<size_t>((size_t)W * W * W, (size_t)kCells * 7);
-    hipLaunchKernelGGL(leaf_kernel<RHO>, dim3(nitems), dim3(512), lds, st, la);
+    if (tab.p2p_cube)
+        hipLaunchKernelGGL((leaf_kernel<RHO, true>), dim3(nitems), dim3(512), lds, st, la);
+    else
+        hipLaunchKernelGGL((leaf_kernel<RHO, false>), dim3(nitems), dim3(512), lds, st, la);
 }
 
 // One target sub-grid is split over 8 / WAVES workgroups of WAVES waves.  The
@@ -1538,9 +1575,9 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         la.neg_inv_dx = -1.0 / dx;
         if (item_flags & kItemLeaf) {
             if (rho)
-                launch_leaf<true>(1, st, la, c->tab.p2p_halo);
+                launch_leaf<true>(1, st, la, c->tab);
             else
-                launch_leaf<false>(1, st, la, c->tab.p2p_halo);
+                launch_leaf<false>(1, st, la, c->tab);
         } else if (rho) {
             launch_level<true>(8, 1, st, la);
         } else {
@@ -1810,9 +1847,9 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
             la.nitems = p->nitems - p->n_m2l;
             la.item0 = p->n_m2l;
             if (rho)
-                launch_leaf<true>(la.nitems, st, la, c->tab.p2p_halo);
+                launch_leaf<true>(la.nitems, st, la, c->tab);
             else
-                launch_leaf<false>(la.nitems, st, la, c->tab.p2p_halo);
+                launch_leaf<false>(la.nitems, st, la, c->tab);
             HIPC(c, hipGetLastError());
         }
     }

@@ -56,6 +56,11 @@ struct Tables {
     // p2p_halo, x slowest), window_offset = (dx * W + dy) * W + dz, and
     // const_byte_offset locates the pair's four constants in p2pconst
     int32_t p2p_halo;
+    // 1 when every class list is the whole 6^3 fine block around the target's
+    // coarse cell minus the target (max |delta| = 1, i.e. theta = 0.5): the
+    // leaf kernel then walks that block with compile-time offsets and takes
+    // the constants straight from `four`
+    int32_t p2p_cube;
     int32_t np2p_k[8];
     int32_t p2plist[8][kMaxDelta * 8];
     // the distinct four-constant rows the eight lists use (342 at theta 0.5),

@@ -174,6 +174,10 @@ bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
             }
         t.np2p_k[k] = q;
     }
+    // with max |delta| = 1 every position 2 delta + j of the 6^3 block lies in
+    // the box, so 215 pairs per class means the block minus the target itself
+    t.p2p_cube = (maxd == 1 && h == 3) ? 1 : 0;
+    for (int k = 0; k < 8; ++k) t.p2p_cube &= (t.np2p_k[k] == 215) ? 1 : 0;
     return true;
 }
 
