@@ -244,7 +244,8 @@ typedef struct octofmm_tree_level {
                                   bit 0 x, bit 1 y, bit 2 z (geo::octant, octotiger/geometry.hpp:333-335) */
     const double* xbase;       /* host [nnodes][3] centre of each node's cell (0,0,0) */
     double dx;                 /* cell width on this level */
-    /* device arrays, AoS per node, caller-owned */
+    /* device arrays, AoS per node, caller-owned; d_L, d_L_c and d_g 16-byte aligned
+       (hipMalloc's are) */
     double* d_M;               /* [nnodes][512][20] written by the upward pass (non-leaf nodes) */
     double* d_com;             /* [nnodes][512][3]  written by the upward pass for RHO (leaf cells:
                                   geometric centres); DRHODT reuses the RHO pass's values */

@@ -133,26 +133,12 @@ __device__ __forceinline__ int partner_base(const LaneGeom& g, int ax, int ay, i
 // cell's centre of mass (taylor::operator<<=, grid::compute_expansions,
 // src/grid_fmm.cpp:1524-1637), L_c + the parent's L_c, and for leaf cells
 // phi = G L0 and g = -G (L_1..3 + L_c).
+// The final expansion of a cell in a whole-tree solve: its interactions plus
+// the parent cell's expansion re-expanded about the cell's centre of mass, and
+// L_c plus the parent's L_c.
 template <bool RHO>
-__device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int cell, const double* L, int ncomp,
-    const double* C, bool write_c, bool leaf_target) {
-    // expansion rows are 160 bytes and 16-byte aligned: moved as 10 double2
-    double2* Lo = reinterpret_cast<double2*>(a.L + ((size_t)slot * kCells + cell) * 20);
-    double* Co = a.Lc + ((size_t)slot * kCells + cell) * 3;
-    if (!a.l2l) {
-        for (int i = 0; i < ncomp / 2; ++i) {
-            double2 v = Lo[i];
-            v.x += L[2 * i];
-            v.y += L[2 * i + 1];
-            Lo[i] = v;
-        }
-        if (RHO && write_c) {
-            Co[0] = C[0];
-            Co[1] = C[1];
-            Co[2] = C[2];
-        }
-        return;
-    }
+__device__ __forceinline__ void tree_final_row(const LevelArgs& a, int slot, int cell, const double* L, int ncomp,
+    const double* C, bool write_c, double* Lf, double* Cf) {
     double l[20], lc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < 20; ++j) l[j] = 0.0;
@@ -178,25 +164,60 @@ __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int ce
             lc[2] = q[2];
         }
     }
-    double Lf[20];
 #pragma unroll
     for (int j = 0; j < 20; ++j) Lf[j] = (j < ncomp ? L[j] : 0.0) + l[j];
 #pragma unroll
+    for (int d = 0; d < 3; ++d) Cf[d] = RHO ? (write_c ? C[d] : 0.0) + lc[d] : 0.0;
+}
+
+// Leaf force assembly: phi = G L0, g = -G (L_1..3 + L_c).
+template <bool RHO>
+__device__ __forceinline__ void leaf_force(const LevelArgs& a, const double* Lf, const double* Cf, double* g4) {
+    g4[0] = a.G * Lf[0];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g4[1 + d] = RHO ? -a.G * Lf[1 + d] - (a.correct_am ? a.G * Cf[d] : 0.0) : 0.0;
+}
+
+// Expansion epilogue of one target cell.  Outside a whole-tree solve:
+// accumulate the cell's L (the ncomp leading components: 20 for M2L
+// targets, 4 for leaves) and overwrite L_c when the kernel forms it
+// (write_c).  In a whole-tree solve the cell's final expansion is written
+// directly: interactions + the parent cell's expansion re-expanded about the
+// cell's centre of mass (taylor::operator<<=, grid::compute_expansions,
+// src/grid_fmm.cpp:1524-1637), L_c + the parent's L_c, and for leaf cells
+// phi = G L0 and g = -G (L_1..3 + L_c).
+template <bool RHO>
+__device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int cell, const double* L, int ncomp,
+    const double* C, bool write_c, bool leaf_target) {
+    // expansion rows are 160 bytes and 16-byte aligned: moved as 10 double2
+    double2* Lo = reinterpret_cast<double2*>(a.L + ((size_t)slot * kCells + cell) * 20);
+    double* Co = a.Lc + ((size_t)slot * kCells + cell) * 3;
+    if (!a.l2l) {
+        for (int i = 0; i < ncomp / 2; ++i) {
+            double2 v = Lo[i];
+            v.x += L[2 * i];
+            v.y += L[2 * i + 1];
+            Lo[i] = v;
+        }
+        if (RHO && write_c) {
+            Co[0] = C[0];
+            Co[1] = C[1];
+            Co[2] = C[2];
+        }
+        return;
+    }
+    double Lf[20], Cf[3];
+    tree_final_row<RHO>(a, slot, cell, L, ncomp, C, write_c, Lf, Cf);
+#pragma unroll
     for (int j = 0; j < 10; ++j) Lo[j] = make_double2(Lf[2 * j], Lf[2 * j + 1]);
-    double Cf[3] = {0.0, 0.0, 0.0};
     if (RHO) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            Cf[d] = (write_c ? C[d] : 0.0) + lc[d];
-            Co[d] = Cf[d];
-        }
+        for (int d = 0; d < 3; ++d) Co[d] = Cf[d];
     }
     if (a.g && leaf_target) {
         double2* go = reinterpret_cast<double2*>(a.g + ((size_t)slot * kCells + cell) * 4);
-        double g4[4] = {a.G * Lf[0], 0.0, 0.0, 0.0};
-        if (RHO)
-#pragma unroll
-            for (int d = 0; d < 3; ++d) g4[1 + d] = -a.G * Lf[1 + d] - (a.correct_am ? a.G * Cf[d] : 0.0);
+        double g4[4];
+        leaf_force<RHO>(a, Lf, Cf, g4);
         if (RHO) {
             go[0] = make_double2(g4[0], g4[1]);
             go[1] = make_double2(g4[2], g4[3]);
@@ -204,6 +225,45 @@ __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int ce
             reinterpret_cast<double*>(go)[0] = g4[0];   // DRHODT: phi slot only
         }
     }
+}
+
+// Leaf epilogue of a whole-tree RHO solve, with the rows staged through LDS so
+// that the sub-grid's L, L_c and g blocks (contiguous: 512 rows of 20, 3 and 4
+// doubles) are written by consecutive lanes in consecutive 16-byte chunks.
+// Written straight from the class waves, each store instruction would touch
+// 64 rows two cells apart.  L goes in two halves of 256 cells (40 KiB of
+// LDS), then L_c and g together (28 KiB).  Every thread of the workgroup
+// calls it; buf is the kernel's dynamic LDS, no longer in use.
+__device__ __forceinline__ void leaf_rows_coalesced(const LevelArgs& a, int slot, int cell, const double* Lf,
+    const double* Cf, const double* g4, double* __restrict__ buf) {
+    const int t = threadIdx.x;
+    double2* b2 = reinterpret_cast<double2*>(buf);
+    for (int half = 0; half < 2; ++half) {
+        __syncthreads();
+        if ((cell >> 8) == half) {
+            double2* row = b2 + (cell & 255) * 10;
+#pragma unroll
+            for (int j = 0; j < 10; ++j) row[j] = make_double2(Lf[2 * j], Lf[2 * j + 1]);
+        }
+        __syncthreads();
+        double2* dst = reinterpret_cast<double2*>(a.L + ((size_t)slot * kCells + half * 256) * 20);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) dst[t + 512 * r] = b2[t + 512 * r];   // 256 rows x 10 chunks
+    }
+    __syncthreads();
+    buf[cell * 3 + 0] = Cf[0];
+    buf[cell * 3 + 1] = Cf[1];
+    buf[cell * 3 + 2] = Cf[2];
+    double2* bg = b2 + 768;
+    bg[cell * 2] = make_double2(g4[0], g4[1]);
+    bg[cell * 2 + 1] = make_double2(g4[2], g4[3]);
+    __syncthreads();
+    double2* lc = reinterpret_cast<double2*>(a.Lc + (size_t)slot * kCells * 3);
+    double2* gg = reinterpret_cast<double2*>(a.g + (size_t)slot * kCells * 4);
+    lc[t] = b2[t];                                    // 768 chunks of L_c
+    if (t < 256) lc[512 + t] = b2[512 + t];
+    gg[t] = bg[t];                                    // 1024 chunks of g
+    gg[512 + t] = bg[512 + t];
 }
 
 template <bool RHO, bool MIXED>
@@ -947,13 +1007,24 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
         C[1] = o[5];
         C[2] = o[6];
     }
+    if constexpr (RHO) {
+        if (a.l2l && a.g) {   // uniform: whole-tree RHO solve
+            double Lf[20], Cf[3], g4[4];
+            tree_final_row<RHO>(a, slot, cell, L, 4, C, p2m, Lf, Cf);
+            leaf_force<RHO>(a, Lf, Cf, g4);
+            leaf_rows_coalesced(a, slot, cell, Lf, Cf, g4, s_win);
+            return;
+        }
+    }
     finish_cell<RHO>(a, slot, cell, L, 4, C, p2m, true);
 }
 
 template <bool RHO>
 void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, const Tables& tab) {
     const int W = kInx + 2 * tab.p2p_halo;
-    const size_t lds = sizeof(double) * std::max<size_t>((size_t)W * W * W, (size_t)kCells * 7);
+    // window, p2m sums (512 x 7), and the staged L rows of half a sub-grid (256 x 20)
+    const size_t lds = sizeof(double) * std::max<size_t>(std::max<size_t>((size_t)W * W * W, (size_t)kCells * 7),
+                                                         (size_t)256 * 20);
     if (tab.p2p_cube)
         hipLaunchKernelGGL((leaf_kernel<RHO, true>), dim3(nitems), dim3(512), lds, st, la);
     else
@@ -2356,6 +2427,9 @@ octofmm_status octofmm_tree_plan_create(octofmm_context* c, const octofmm_tree* 
             return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad description of level %d", l);
         if (!L.d_M || !L.d_com || !L.d_mon || !L.d_L || (rho && !L.d_L_c))
             return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "null device array on level %d", l);
+        // expansion, correction and force rows are moved in 16-byte pieces
+        if (((uintptr_t)L.d_L | (uintptr_t)L.d_L_c | (uintptr_t)L.d_g) & 15)
+            return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "L, L_c and g of level %d must be 16-byte aligned", l);
         if (l == 0 && (L.nnodes != 1 || L.parent[0] != -1))
             return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "level 0 must hold the root alone, without a parent");
         std::vector<int> children(l + 1 < nl ? tr->levels[l + 1].nnodes : 0, 0);
