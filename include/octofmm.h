@@ -210,12 +210,17 @@ typedef struct octofmm_plan_stats {
 
 octofmm_status octofmm_level_plan_create(octofmm_context* ctx, const octofmm_level* level,
     octofmm_plan** plan);
-/* Executes the plan on a HIP stream (hipStream_t, NULL = default stream); asynchronous. */
+/* Executes the plan on a HIP stream (hipStream_t, NULL = default stream); asynchronous.
+ * A level with both multipole and leaf targets runs its leaf kernel on a
+ * plan-owned stream, forked after staging and joined back into `stream`, so
+ * the pass is still ordered like one launch on `stream`.  Executes of one plan
+ * must not overlap. */
 octofmm_status octofmm_level_plan_execute(octofmm_plan* plan, void* stream);
 octofmm_status octofmm_level_plan_stats(const octofmm_plan* plan, octofmm_plan_stats* stats);
 /* Runs each stage of the plan `iters` times on `stream`, bracketed by HIP
  * events recorded on that stream, and writes the average milliseconds per
- * stage: ms[0] AoS->SoA staging, ms[1] interaction kernel, ms[2] write-back. */
+ * stage: ms[0] AoS->SoA staging, ms[1] interaction kernels (both kinds,
+ * overlapped as in execute), ms[2] = 0 (write-back is fused). */
 octofmm_status octofmm_level_plan_time_stages(octofmm_plan* plan, void* stream, int32_t iters,
     double* ms);
 octofmm_status octofmm_level_plan_destroy(octofmm_plan* plan);

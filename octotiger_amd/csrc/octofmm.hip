@@ -1340,6 +1340,10 @@ struct octofmm_plan {
     double* d_partial = nullptr;   // root: per-chunk partial expansions
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // a level with both kinds of items runs its leaf kernel on this stream,
+    // forked after staging and joined before the pass ends
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -1822,6 +1826,9 @@ static void plan_free(octofmm_plan* p) {
     if (!p) return;
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->side) (void)hipStreamDestroy(p->side);
     (void)hipFree(p->d_rec);
     (void)hipFree(p->d_nbr);
     (void)hipFree(p->d_kind);
@@ -1924,6 +1931,40 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
             HIPC(c, hipGetLastError());
         }
     }
+    return OCTOFMM_OK;
+}
+
+// The kernels of one pass over the masked stages.  A level with both kinds of
+// items runs its leaf kernel (bit 2) on the plan's side stream, forked after
+// the staging and joined back before returning: the two kernels write
+// disjoint sub-grids, so the leaf kernel overlaps the multipole-target
+// kernel instead of queueing behind it.
+static bool plan_forks(const octofmm_plan* p, int mask) {
+    return !p->root && p->n_m2l > 0 && p->nitems > p->n_m2l && (mask & 6) == 6;
+}
+
+static octofmm_status plan_fork_resources(octofmm_plan* p) {
+    octofmm_context* c = p->ctx;
+    if (p->side || !plan_forks(p, 7)) return OCTOFMM_OK;
+    HIPC(c, hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+    HIPC(c, hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+    HIPC(c, hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+    return OCTOFMM_OK;
+}
+
+static octofmm_status plan_run(octofmm_plan* p, hipStream_t st, int mask) {
+    octofmm_context* c = p->ctx;
+    if (!plan_forks(p, mask) || !p->side) return plan_launch(p, st, mask);
+    octofmm_status s = plan_launch(p, st, mask & 1);
+    if (s != OCTOFMM_OK) return s;
+    HIPC(c, hipEventRecord(p->ev_fork, st));
+    HIPC(c, hipStreamWaitEvent(p->side, p->ev_fork, 0));
+    s = plan_launch(p, p->side, 4);
+    if (s != OCTOFMM_OK) return s;
+    s = plan_launch(p, st, 2);
+    if (s != OCTOFMM_OK) return s;
+    HIPC(c, hipEventRecord(p->ev_join, p->side));
+    HIPC(c, hipStreamWaitEvent(st, p->ev_join, 0));
     return OCTOFMM_OK;
 }
 
@@ -2212,10 +2253,18 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
     }
     p->stats.variant = p->variant;
     if (lv->flags & OCTOFMM_PLAN_USE_GRAPH) {
+        // the side stream must exist before the capture starts
+        const octofmm_status fs = plan_fork_resources(p);
+        if (fs != OCTOFMM_OK) {
+            plan_free(p);
+            return fs;
+        }
+    }
+    if (lv->flags & OCTOFMM_PLAN_USE_GRAPH) {
         hipStream_t cs = c->stream;
         if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return cleanup(OCTOFMM_ERR_HIP, "graph capture failed to start");
-        octofmm_status ls = plan_launch(p, cs, 7);
+        octofmm_status ls = plan_run(p, cs, 7);
         hipGraph_t g = nullptr;
         const hipError_t ee = hipStreamEndCapture(cs, &g);
         if (ls != OCTOFMM_OK || ee != hipSuccess) {
@@ -2239,7 +2288,9 @@ octofmm_status octofmm_level_plan_execute(octofmm_plan* p, void* stream) {
         HIPC(c, hipGraphLaunch(p->gexec, st));
         return OCTOFMM_OK;
     }
-    return plan_launch(p, st, 7);
+    const octofmm_status fs = plan_fork_resources(p);   // first eager execute
+    if (fs != OCTOFMM_OK) return fs;
+    return plan_run(p, st, 7);
 }
 
 octofmm_status octofmm_level_plan_stats(const octofmm_plan* p, octofmm_plan_stats* s) {
@@ -2253,13 +2304,15 @@ octofmm_status octofmm_level_plan_time_stages(octofmm_plan* p, void* stream, int
     octofmm_context* c = p->ctx;
     HIPC(c, hipSetDevice(c->device));
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const octofmm_status fs = plan_fork_resources(p);
+    if (fs != OCTOFMM_OK) return fs;
     hipEvent_t e0, e1;
     HIPC(c, hipEventCreate(&e0));
     HIPC(c, hipEventCreate(&e1));
     for (int stage = 0; stage < 2; ++stage) {
         HIPC(c, hipEventRecord(e0, st));
         for (int i = 0; i < iters; ++i) {
-            octofmm_status s = plan_launch(p, st, stage == 0 ? 1 : 6);
+            octofmm_status s = plan_run(p, st, stage == 0 ? 1 : 6);
             if (s != OCTOFMM_OK) return s;
         }
         HIPC(c, hipEventRecord(e1, st));

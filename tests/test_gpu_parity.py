@@ -114,10 +114,15 @@ def test_level_uniform_small(ctx, t, graph):
 
 
 @pytest.mark.parametrize("t", [_abi.RHO, _abi.DRHODT])
-def test_level_mixed_kinds(ctx, t):
+@pytest.mark.parametrize("graph", [True, False])
+def test_level_mixed_kinds(ctx, t, graph):
+    """Both kinds of items: the leaf kernel runs on the plan's side stream
+    beside the multipole-target kernel, in the graph and in direct launches;
+    two executes are bit-identical."""
     lv = workloads.mixed_level((6, 6, 6), seed=11, leaf_fraction=0.5)
-    print(f"mixed level {lv.info} type {t}", flush=True)
-    outs, stats = run_level(ctx, lv, t)
+    print(f"mixed level {lv.info} type {t} graph {graph}", flush=True)
+    outs, stats = run_level(ctx, lv, t, _abi.PLAN_USE_GRAPH if graph else 0, repeats=2)
+    assert all(np.array_equal(a, b) for a, b in zip(outs[0], outs[1]))
     L, Lc = outs[0]
     Lr, Lcr = oracle.level_interactions(lv, type_=t)
     tg = lv.targets
