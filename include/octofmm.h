@@ -115,6 +115,8 @@ typedef struct octofmm_context octofmm_context;
  * init_fmm_constants (src/cuda_util/cuda_scheduler.cpp:31-90).  A context may be
  * used by one host thread at a time. */
 octofmm_status octofmm_create(const octofmm_config* config, octofmm_context** ctx);
+/* Destroy every level and tree plan made from a context before the context:
+ * plans use it until they are destroyed. */
 octofmm_status octofmm_destroy(octofmm_context* ctx);
 const char* octofmm_status_string(octofmm_status status);
 /* Text of the last error recorded on the context (never NULL). */

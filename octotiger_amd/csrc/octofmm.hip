@@ -1776,6 +1776,7 @@ octofmm_status octofmm_compute_multipole_interactions(octofmm_context* c, const 
     octofmm_status s = check_finite_inputs(c, in, false);
     if (s != OCTOFMM_OK) return s;
     HIPC(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // launch checks below report this call's launches only
     s = ensure_workspace(c);
     if (s != OCTOFMM_OK) return s;
     s = fill_workspace(c, in, false, root);
@@ -1805,6 +1806,7 @@ octofmm_status octofmm_compute_monopole_interactions(octofmm_context* c, const o
     octofmm_status s = check_finite_inputs(c, in, true);
     if (s != OCTOFMM_OK) return s;
     HIPC(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // launch checks below report this call's launches only
     s = ensure_workspace(c);
     if (s != OCTOFMM_OK) return s;
     s = fill_workspace(c, in, true, false);
@@ -2151,6 +2153,7 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
     }
 
     HIPC(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // launch checks below report this call's launches only
     octofmm_plan* p = new octofmm_plan();
     p->ctx = c;
     p->nslots = ns;
@@ -2283,6 +2286,7 @@ octofmm_status octofmm_level_plan_execute(octofmm_plan* p, void* stream) {
     if (!p) return OCTOFMM_ERR_INVALID_ARGUMENT;
     octofmm_context* c = p->ctx;
     HIPC(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // launch checks below report this call's launches only
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if (p->gexec) {
         HIPC(c, hipGraphLaunch(p->gexec, st));
@@ -2303,6 +2307,7 @@ octofmm_status octofmm_level_plan_time_stages(octofmm_plan* p, void* stream, int
     if (!p || !ms || iters <= 0) return OCTOFMM_ERR_INVALID_ARGUMENT;
     octofmm_context* c = p->ctx;
     HIPC(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // launch checks below report this call's launches only
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const octofmm_status fs = plan_fork_resources(p);
     if (fs != OCTOFMM_OK) return fs;
@@ -2515,6 +2520,7 @@ octofmm_status octofmm_tree_plan_create(octofmm_context* c, const octofmm_tree* 
         }
     }
     HIPC(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // launch checks below report this call's launches only
     octofmm_tree_plan* t = new octofmm_tree_plan();
     t->ctx = c;
     t->type = tr->type;
@@ -2651,6 +2657,7 @@ octofmm_status octofmm_tree_plan_execute(octofmm_tree_plan* t, void* stream) {
     if (!t) return OCTOFMM_ERR_INVALID_ARGUMENT;
     octofmm_context* c = t->ctx;
     HIPC(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // launch checks below report this call's launches only
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if (t->gexec) {
         HIPC(c, hipGraphLaunch(t->gexec, st));
@@ -2669,6 +2676,7 @@ octofmm_status octofmm_tree_plan_time_phases(octofmm_tree_plan* t, void* stream,
     if (!t || !ms || iters <= 0) return OCTOFMM_ERR_INVALID_ARGUMENT;
     octofmm_context* c = t->ctx;
     HIPC(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // launch checks below report this call's launches only
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipEvent_t e0, e1;
     HIPC(c, hipEventCreate(&e0));
@@ -2695,6 +2703,7 @@ octofmm_status octofmm_measure_fma_rate(octofmm_context* c, int32_t precision, d
     if (precision != OCTOFMM_PRECISION_FP64 && precision != OCTOFMM_PRECISION_MIXED)
         return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "precision must be FP64 or MIXED (FP32)");
     HIPC(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // launch checks below report this call's launches only
     // 4096 workgroups of 256 lanes (16 per CU on 256 CUs), 8 chains x 2048 steps each
     constexpr int kBlocks = 4096, kIters = 2048;
     void* out = nullptr;

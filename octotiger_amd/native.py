@@ -5,6 +5,7 @@ present, every call raises -- there is no CPU fallback.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -82,13 +83,18 @@ class Context:
             raise OctofmmError(st, lib().octofmm_status_string(st).decode())
         self.theta = theta
         self.device = device
+        self._plans = weakref.WeakSet()   # plans hold the context: closed before it
 
     def _check(self, st):
         if st != _abi.OK:
             raise OctofmmError(st, lib().octofmm_last_error(self._h).decode())
 
     def close(self):
+        """Destroys the context, after every plan made from it that is still
+        open (a plan must not outlive its context)."""
         if self._h:
+            for p in list(self._plans):
+                p.close()
             lib().octofmm_destroy(self._h)
             self._h = ctypes.c_void_p()
 
@@ -162,6 +168,7 @@ class TreePlan:
         tr.correct_am = 1 if correct_am else 0
         self._h = ctypes.c_void_p()
         ctx._check(lib().octofmm_tree_plan_create(ctx._h, ctypes.byref(tr), ctypes.byref(self._h)))
+        ctx._plans.add(self)
 
     def execute(self, stream=0):
         self.ctx._check(lib().octofmm_tree_plan_execute(self._h, ctypes.c_void_p(stream)))
@@ -212,6 +219,7 @@ class LevelPlan:
         lv.d_M, lv.d_com, lv.d_mon, lv.d_L, lv.d_L_c = d_M, d_com, d_mon, d_L, d_L_c
         self._h = ctypes.c_void_p()
         ctx._check(lib().octofmm_level_plan_create(ctx._h, ctypes.byref(lv), ctypes.byref(self._h)))
+        ctx._plans.add(self)
 
     def execute(self, stream=0):
         self.ctx._check(lib().octofmm_level_plan_execute(self._h, ctypes.c_void_p(stream)))

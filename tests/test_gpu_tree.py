@@ -86,6 +86,23 @@ def test_small_tree_matches_oracle(ctx):
     dev.free()
 
 
+@pytest.mark.parametrize("theta", [0.4, 0.34])
+def test_small_tree_other_theta_matches_oracle(theta):
+    """Below theta = 0.5 the stencil reaches two coarse cells: the leaf kernel
+    takes its list-driven p2p path over a wider window, with the same
+    contiguous tree epilogue."""
+    c = native.Context(theta=theta, device=0)
+    t = workloads.small_tree()
+    print(f"tree {t.name} theta {theta}", flush=True)
+    plan, dev, got = run(c, t)
+    compare(t, got, oracle.tree_solve(t, theta=theta))
+    ef, et = totals(t, got)
+    assert ef < 1e-13 and et < 1e-13
+    plan.close()
+    dev.free()
+    c.close()
+
+
 def test_tree_replays_clear_and_repeat(ctx):
     t = workloads.small_tree()
     plan, dev, first = run(ctx, t)
