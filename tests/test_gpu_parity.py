@@ -133,13 +133,17 @@ def test_level_mixed_kinds(ctx, t, graph):
 
 
 @pytest.mark.parametrize("theta", [0.34, 0.45])
-def test_level_mixed_kinds_other_theta(theta):
-    """Wider stencils (theta 0.34: coarse offsets up to 2, p2p window 18^3)."""
+@pytest.mark.parametrize("variant", [8, 108, 208, 304, 404, 504])
+def test_level_mixed_kinds_other_theta(theta, variant, monkeypatch):
+    """Wider stencils (theta 0.34: coarse offsets up to 2, p2p window 18^3)
+    under every lane mapping, the staged ones included."""
+    monkeypatch.setenv("OCTOFMM_VARIANT", str(variant))
     c = native.Context(theta=theta, device=0)
     lv = workloads.mixed_level((5, 5, 5), seed=13, leaf_fraction=0.5)
-    print(f"mixed level theta {theta} {lv.info}", flush=True)
+    print(f"mixed level theta {theta} variant {variant} {lv.info}", flush=True)
     outs, stats = run_level(c, lv, _abi.RHO)
     c.close()
+    assert stats["variant"] == variant
     L, Lc = outs[0]
     Lr, Lcr = oracle.level_interactions(lv, theta=theta, type_=_abi.RHO)
     tg = lv.targets
