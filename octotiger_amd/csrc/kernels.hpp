@@ -34,6 +34,27 @@ __host__ __device__ constexpr double record_scale(int c) {
         : 1.0;
 }
 
+// Sum over the 8 lanes of an aligned group, by three DPP lane moves (VALU ops,
+// no LDS round trip): the pair of xor 1, the pair of xor 2 within each quad,
+// then the mirrored lane of the other quad (7 - i within each half row, once
+// every lane holds its quad's sum).  Every lane ends with the same value:
+// ((v0 + v1) + (v2 + v3)) + ((v4 + v5) + (v6 + v7)), in the same order on
+// every run.  The 8 lanes of a group must be active together.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double sum8(double v) {
+    v += dpp_move<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_move<0x141>(v);   // row_half_mirror
+    return v;
+}
+
 struct DTensor {
     double D[20];
     double d2, d3;

@@ -987,9 +987,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
                 for (int e = sub; e < te.z; e += 8)
                     p2m_pair<RHO>(base + (size_t)a.p2m_part[te.y + e] * kRecDoubles, X0, X1, X2, v, v + 4);
 #pragma unroll
-                for (int m = 1; m < 8; m <<= 1)
-#pragma unroll
-                    for (int i = 0; i < 7; ++i) v[i] += __shfl_xor(v[i], m, 8);
+                for (int i = 0; i < 7; ++i) v[i] = sum8(v[i]);
                 if (sub == 0) {
                     double* o = acc + te.x * 7;
 #pragma unroll
@@ -1201,12 +1199,7 @@ __device__ __forceinline__ double cell_centre(double xb, int i, double dx) {
 // form the coarse cell's mass and mass-weighted centre by a fixed butterfly,
 // shift their child's multipole to it, and sum the shifted multipoles the
 // same way; lane 0 of the group writes the parent's cell.
-__device__ __forceinline__ double sum8(double v) {
-    v += __shfl_xor(v, 1, 8);
-    v += __shfl_xor(v, 2, 8);
-    v += __shfl_xor(v, 4, 8);
-    return v;
-}
+
 
 __global__ __launch_bounds__(256) void m2m_kernel(M2MArgs a) {
     const int gid = blockIdx.x * 256 + threadIdx.x;
