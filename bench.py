@@ -267,6 +267,14 @@ def main():
                              "with g++ -O2, walked in the Kokkos host-kernel order with SIMD width 1"
                              if use_ref else "FP64 scalar oracle (restated Kokkos host-kernel walk)")
                           + ", one sub-grid per OpenMP task")}
+        # one core: the first 32 targets of the level on one thread
+        one = lv.targets[:32]
+        t = time.perf_counter()
+        oracle.level_interactions(lv, targets=one, nthreads=1, use_ref=use_ref)
+        dt1 = time.perf_counter() - t
+        pairs1 = pairs * len(one) / len(lv.targets)
+        cpu["single_core_value"] = pairs1 / dt1
+        cpu["single_core_sample"] = f"{len(one)} sub-grids ({int(pairs1)} pairs) on 1 thread, {dt1:.3f} s"
 
     mixed = {}
     m2 = c2["mixed"]
