@@ -439,6 +439,12 @@ struct LevelArgs {
     const int32_t* p2m_dir;
     const int4* p2m_tgt;
     const int32_t* p2m_part;
+    // per direction, the union of its partner records (the slab staged in LDS
+    // on the theta = 0.5 path): [27][2] (count, offset), the record indices,
+    // and the partners as slab indices (same layout as p2m_part)
+    const int32_t* p2m_sdir;
+    const int32_t* p2m_slab;
+    const int32_t* p2m_spart;
     // Whole-tree solves (l2l != 0): the level's expansions are written, not
     // accumulated, as interactions + the parent cell's final expansion
     // re-expanded about the cell's centre of mass (and L_c + the parent's

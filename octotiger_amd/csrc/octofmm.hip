@@ -878,6 +878,8 @@ __device__ void m2l_item_classwave(const LevelArgs& a, int slot, int flags, cons
 // the 1/dx and -1/dx^2 factors are applied once per cell).  Refined
 // neighbours then contribute through p2m with their full multipoles over the
 // same stencil, far and near alike (monopole_kernel_templates.hpp:69-239).
+constexpr int kP2mSlab = 128;   // records of one direction's p2m slab in LDS (theta = 0.5: 128)
+
 template <bool RHO, bool CUBE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void leaf_kernel(LevelArgs a) {
     extern __shared__ double s_win[];
@@ -970,6 +972,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
         // directions, so every target's sum is formed in the same order on
         // every run.
         double* acc = s_win;   // the p2p window is no longer needed
+        // theta = 0.5: the direction's partner records (at most kP2mSlab: two
+        // fine layers of a face) are first copied into LDS by the whole
+        // workgroup, all loads in flight at once, and the pairs read them from
+        // there instead of waiting on one global load per pair
+        double* slab = s_win + kCells * 7;
         __syncthreads();
         for (int i = threadIdx.x; i < kCells * 7; i += 512) acc[i] = 0.0;
         __syncthreads();
@@ -977,15 +984,25 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
             const int ns = s_nbr[r];
             if (r == 13 || ns < 0 || s_kind[r] != OCTOFMM_SLOT_NONLEAF) continue;
             const int nt = a.p2m_dir[2 * r], t0 = a.p2m_dir[2 * r + 1];
+            const double* base = a.rec + (size_t)ns * kCells * kRecDoubles;
+            if constexpr (CUBE) {
+                const int sn = a.p2m_sdir[2 * r], so = a.p2m_sdir[2 * r + 1];
+                const double2* src = reinterpret_cast<const double2*>(base);
+                double2* dst = reinterpret_cast<double2*>(slab);
+                for (int c2 = threadIdx.x; c2 < sn * 12; c2 += 512)
+                    dst[c2] = src[(size_t)a.p2m_slab[so + c2 / 12] * 12 + c2 % 12];
+                __syncthreads();
+            }
+            const double* pbase = CUBE ? slab : base;
+            const int32_t* pidx = CUBE ? a.p2m_spart : a.p2m_part;
             for (int q = threadIdx.x; q < nt * 8; q += 512) {
                 const int sub = q & 7;
                 const int4 te = a.p2m_tgt[t0 + (q >> 3)];
                 const double* xc = a.com + ((size_t)slot * kCells + te.x) * 3;
                 const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
                 double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                const double* base = a.rec + (size_t)ns * kCells * kRecDoubles;
                 for (int e = sub; e < te.z; e += 8)
-                    p2m_pair<RHO>(base + (size_t)a.p2m_part[te.y + e] * kRecDoubles, X0, X1, X2, v, v + 4);
+                    p2m_pair<RHO>(pbase + (size_t)pidx[te.y + e] * kRecDoubles, X0, X1, X2, v, v + 4);
 #pragma unroll
                 for (int i = 0; i < 7; ++i) v[i] = sum8(v[i]);
                 if (sub == 0) {
@@ -1018,12 +1035,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
 }
 
 template <bool RHO>
-void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, const Tables& tab) {
+void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, const Tables& tab, int slab_max) {
     const int W = kInx + 2 * tab.p2p_halo;
-    // window, p2m sums (512 x 7), and the staged L rows of half a sub-grid (256 x 20)
-    const size_t lds = sizeof(double) * std::max<size_t>(std::max<size_t>((size_t)W * W * W, (size_t)kCells * 7),
-                                                         (size_t)256 * 20);
-    if (tab.p2p_cube)
+    const bool cube = tab.p2p_cube && slab_max <= kP2mSlab;
+    // window, p2m sums (512 x 7) with the p2m slab on the cube path, and the
+    // staged L rows of half a sub-grid (256 x 20)
+    const size_t p2m = (size_t)kCells * 7 + (cube ? (size_t)kP2mSlab * kRecDoubles : 0);
+    const size_t lds = sizeof(double) * std::max<size_t>(std::max<size_t>((size_t)W * W * W, p2m), (size_t)256 * 20);
+    if (cube)
         hipLaunchKernelGGL((leaf_kernel<RHO, true>), dim3(nitems), dim3(512), lds, st, la);
     else
         hipLaunchKernelGGL((leaf_kernel<RHO, false>), dim3(nitems), dim3(512), lds, st, la);
@@ -1296,6 +1315,10 @@ struct octofmm_context {
     int32_t* d_p2m_dir = nullptr;
     int4* d_p2m_tgt = nullptr;
     int32_t* d_p2m_part = nullptr;
+    int32_t* d_p2m_sdir = nullptr;
+    int32_t* d_p2m_slab = nullptr;
+    int32_t* d_p2m_spart = nullptr;
+    int32_t p2m_slab_max = 0;      // largest per-direction slab (records)
 };
 
 struct octofmm_plan {
@@ -1510,7 +1533,8 @@ octofmm_status fill_workspace(octofmm_context* c, const octofmm_subgrid* in, boo
 // stencil reaches into it and, per target, the partner records there, in
 // (coarse offset, child) order.
 void build_p2m_lists(const Tables& T, std::vector<int32_t>& dir, std::vector<int4>& tgt,
-    std::vector<int32_t>& part) {
+    std::vector<int32_t>& part, std::vector<int32_t>& sdir, std::vector<int32_t>& slab,
+    std::vector<int32_t>& spart, int& slab_max) {
     std::vector<int32_t> lists[27][kCells];
     for (int t = 0; t < kCells; ++t) {
         const int x = t >> 6, y = (t >> 3) & 7, z = t & 7;
@@ -1541,6 +1565,28 @@ void build_p2m_lists(const Tables& T, std::vector<int32_t>& dir, std::vector<int
             part.insert(part.end(), lists[r][t].begin(), lists[r][t].end());
         }
         dir[2 * r] = (int32_t)tgt.size() - dir[2 * r + 1];
+    }
+    // per direction, the union of the partner records, ascending, and every
+    // partner re-indexed into it
+    sdir.assign(54, 0);
+    slab.clear();
+    spart.assign(part.size(), 0);
+    slab_max = 0;
+    for (int r = 0; r < 27; ++r) {
+        std::vector<int> pos(kCells, -1);
+        for (int t = 0; t < kCells; ++t)
+            for (int rec : lists[r][t]) pos[rec] = 0;
+        sdir[2 * r + 1] = (int32_t)slab.size();
+        int n = 0;
+        for (int rec = 0; rec < kCells; ++rec)
+            if (pos[rec] == 0) {
+                pos[rec] = n++;
+                slab.push_back(rec);
+            }
+        sdir[2 * r] = n;
+        slab_max = std::max(slab_max, n);
+        for (int q = dir[2 * r + 1]; q < dir[2 * r + 1] + dir[2 * r]; ++q)
+            for (int e = 0; e < tgt[q].z; ++e) spart[tgt[q].y + e] = pos[part[tgt[q].y + e]];
     }
 }
 
@@ -1630,6 +1676,9 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         la.p2m_dir = c->d_p2m_dir;
         la.p2m_tgt = c->d_p2m_tgt;
         la.p2m_part = c->d_p2m_part;
+        la.p2m_sdir = c->d_p2m_sdir;
+        la.p2m_slab = c->d_p2m_slab;
+        la.p2m_spart = c->d_p2m_spart;
         la.parent = nullptr;
         la.octant = nullptr;
         la.pL = la.pLc = la.pcom = nullptr;
@@ -1643,9 +1692,9 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         la.neg_inv_dx = -1.0 / dx;
         if (item_flags & kItemLeaf) {
             if (rho)
-                launch_leaf<true>(1, st, la, c->tab);
+                launch_leaf<true>(1, st, la, c->tab, c->p2m_slab_max);
             else
-                launch_leaf<false>(1, st, la, c->tab);
+                launch_leaf<false>(1, st, la, c->tab, c->p2m_slab_max);
         } else if (rho) {
             launch_level<true>(8, 1, st, la);
         } else {
@@ -1704,9 +1753,9 @@ octofmm_status octofmm_create(const octofmm_config* cfg, octofmm_context** out) 
         octofmm_destroy(c);
         return OCTOFMM_ERR_HIP;
     }
-    std::vector<int32_t> pdir, ppart;
+    std::vector<int32_t> pdir, ppart, psdir, pslab, pspart;
     std::vector<int4> ptgt;
-    build_p2m_lists(c->tab, pdir, ptgt, ppart);
+    build_p2m_lists(c->tab, pdir, ptgt, ppart, psdir, pslab, pspart, c->p2m_slab_max);
     if (hipMalloc((void**)&c->d_p2m_dir, sizeof(int32_t) * pdir.size()) != hipSuccess ||
         hipMalloc((void**)&c->d_p2m_tgt, sizeof(int4) * std::max<size_t>(1, ptgt.size())) != hipSuccess ||
         hipMalloc((void**)&c->d_p2m_part, sizeof(int32_t) * std::max<size_t>(1, ppart.size())) != hipSuccess ||
@@ -1714,7 +1763,15 @@ octofmm_status octofmm_create(const octofmm_config* cfg, octofmm_context** out) 
         (!ptgt.empty() &&
             hipMemcpy(c->d_p2m_tgt, ptgt.data(), sizeof(int4) * ptgt.size(), hipMemcpyHostToDevice) != hipSuccess) ||
         (!ppart.empty() &&
-            hipMemcpy(c->d_p2m_part, ppart.data(), sizeof(int32_t) * ppart.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+            hipMemcpy(c->d_p2m_part, ppart.data(), sizeof(int32_t) * ppart.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMalloc((void**)&c->d_p2m_sdir, sizeof(int32_t) * psdir.size()) != hipSuccess ||
+        hipMalloc((void**)&c->d_p2m_slab, sizeof(int32_t) * std::max<size_t>(1, pslab.size())) != hipSuccess ||
+        hipMalloc((void**)&c->d_p2m_spart, sizeof(int32_t) * std::max<size_t>(1, pspart.size())) != hipSuccess ||
+        hipMemcpy(c->d_p2m_sdir, psdir.data(), sizeof(int32_t) * psdir.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        (!pslab.empty() &&
+            hipMemcpy(c->d_p2m_slab, pslab.data(), sizeof(int32_t) * pslab.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+        (!pspart.empty() &&
+            hipMemcpy(c->d_p2m_spart, pspart.data(), sizeof(int32_t) * pspart.size(), hipMemcpyHostToDevice) != hipSuccess)) {
         octofmm_destroy(c);
         return OCTOFMM_ERR_HIP;
     }
@@ -1745,6 +1802,9 @@ octofmm_status octofmm_destroy(octofmm_context* c) {
     (void)hipFree(c->d_p2m_dir);
     (void)hipFree(c->d_p2m_tgt);
     (void)hipFree(c->d_p2m_part);
+    (void)hipFree(c->d_p2m_sdir);
+    (void)hipFree(c->d_p2m_slab);
+    (void)hipFree(c->d_p2m_spart);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OCTOFMM_OK;
@@ -1895,6 +1955,9 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         la.p2m_dir = c->d_p2m_dir;
         la.p2m_tgt = c->d_p2m_tgt;
         la.p2m_part = c->d_p2m_part;
+        la.p2m_sdir = c->d_p2m_sdir;
+        la.p2m_slab = c->d_p2m_slab;
+        la.p2m_spart = c->d_p2m_spart;
         la.parent = p->t_parent;
         la.octant = p->t_octant;
         la.pL = p->t_pL;
@@ -1920,9 +1983,9 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
             la.nitems = p->nitems - p->n_m2l;
             la.item0 = p->n_m2l;
             if (rho)
-                launch_leaf<true>(la.nitems, st, la, c->tab);
+                launch_leaf<true>(la.nitems, st, la, c->tab, c->p2m_slab_max);
             else
-                launch_leaf<false>(la.nitems, st, la, c->tab);
+                launch_leaf<false>(la.nitems, st, la, c->tab, c->p2m_slab_max);
             HIPC(c, hipGetLastError());
         }
     }
