@@ -48,47 +48,69 @@ __device__ __forceinline__ int xcd_item(int b, int n) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// One block per 128 cells of a slot (two x-planes).  Those cells' records are
+// the 128 consecutive records of the block's coarse x-layer, so the block
+// loads the cells' M and centre-of-mass rows (contiguous) into LDS in 16-byte
+// pieces, then writes the 128 records as one contiguous range, consecutive
+// threads in consecutive 16-byte chunks.  Loads and stores are both
+// coalesced, where a lane per record spread every instruction over 64 rows.
 __global__ __launch_bounds__(256) void stage_records(StageArgs a) {
     // leaf-cell centres are rounded operation by operation, as the reference does
 #pragma clang fp contract(off)
-    const int gid = blockIdx.x * 256 + threadIdx.x;
-    if (gid >= a.nslots * kCells) return;
-    const int slot = a.list ? a.list[gid >> 9] : gid >> 9, cell = gid & 511;
-    const int x = cell >> 6, y = (cell >> 3) & 7, z = cell & 7;
-    const int r = (((x >> 1) << 4) | ((y >> 1) << 2) | (z >> 1)) * 8 + (((x & 1) << 2) | ((y & 1) << 1) | (z & 1));
-    double* o = a.rec + ((size_t)slot * kCells + r) * kRecDoubles;
+    __shared__ double sM[128 * 20];
+    __shared__ double sC[128 * 3];
+    __shared__ double sm[128];
+    const int si = blockIdx.x >> 2, q = blockIdx.x & 3;
+    if (si >= a.nslots) return;                // uniform per block
+    const int slot = a.list ? a.list[si] : si;
     const int k = a.kind[slot];
-    double v[kRecDoubles];
-    if (k == OCTOFMM_SLOT_LEAF) {
-        const double m = a.mon[(size_t)slot * kCells + cell];
-        v[0] = m;
-#pragma unroll
-        for (int i = 1; i < 20; ++i) v[i] = 0.0;
-        // geometric centre in the target's frame (multipole_interaction_interface.hpp:216-235)
-        const int* fo = a.foff + 3 * slot;
-        const double* fb = a.fxbase + 3 * slot;
-        v[20] = (double)(x + fo[0]) * a.dx + fb[0] - kInx * a.dx;
-        v[21] = (double)(y + fo[1]) * a.dx + fb[1] - kInx * a.dx;
-        v[22] = (double)(z + fo[2]) * a.dx + fb[2] - kInx * a.dx;
-        v[23] = m;
-    } else if (k == OCTOFMM_SLOT_NONLEAF) {
-        const double* Mi = a.M + ((size_t)slot * kCells + cell) * 20;
-#pragma unroll
-        for (int i = 0; i < 20; ++i) v[i] = Mi[i];
-        const double* ci = a.com + ((size_t)slot * kCells + cell) * 3;
-        v[20] = ci[0];
-        v[21] = ci[1];
-        v[22] = ci[2];
-        v[23] = 0.0;
-    } else {
-#pragma unroll
-        for (int i = 0; i < kRecDoubles; ++i) v[i] = 0.0;
+    const int c0 = q * 128;
+    const int t = threadIdx.x;
+    const size_t row0 = (size_t)slot * kCells + c0;
+    if (k == OCTOFMM_SLOT_NONLEAF) {
+        const double2* srcM = reinterpret_cast<const double2*>(a.M + row0 * 20);
+        for (int i = t; i < 128 * 10; i += 256) reinterpret_cast<double2*>(sM)[i] = srcM[i];
+        const double* srcC = a.com + row0 * 3;
+        for (int i = t; i < 128 * 3; i += 256) sC[i] = srcC[i];
+    } else if (k == OCTOFMM_SLOT_LEAF) {
+        if (t < 128) sm[t] = a.mon[row0 + t];
     }
-#pragma unroll
-    for (int i = 4; i < 20; ++i) v[i] *= record_scale(i);
-    double2* o2 = reinterpret_cast<double2*>(o);
-#pragma unroll
-    for (int i = 0; i < kRecDoubles / 2; ++i) o2[i] = make_double2(v[2 * i], v[2 * i + 1]);
+    __syncthreads();
+    double2* dst = reinterpret_cast<double2*>(a.rec + row0 * kRecDoubles);
+    for (int ch = t; ch < 128 * 12; ch += 256) {
+        const int r = c0 + ch / 12, part = ch % 12;
+        // the cell of record r: coarse cell r >> 3 (x-major), child r & 7
+        const int co = r >> 3, ci = r & 7;
+        const int x = 2 * (co >> 4) + ((ci >> 2) & 1), y = 2 * ((co >> 2) & 3) + ((ci >> 1) & 1),
+                  z = 2 * (co & 3) + (ci & 1);
+        const int cl = x * 64 + y * 8 + z - c0;
+        double2 v = make_double2(0.0, 0.0);
+        if (k == OCTOFMM_SLOT_NONLEAF) {
+            if (part < 10) {
+                v.x = sM[cl * 20 + 2 * part] * record_scale(2 * part);
+                v.y = sM[cl * 20 + 2 * part + 1] * record_scale(2 * part + 1);
+            } else if (part == 10) {
+                v = make_double2(sC[cl * 3], sC[cl * 3 + 1]);
+            } else {
+                v = make_double2(sC[cl * 3 + 2], 0.0);
+            }
+        } else if (k == OCTOFMM_SLOT_LEAF) {
+            const double m = sm[cl];
+            // geometric centre in the target's frame (multipole_interaction_interface.hpp:216-235)
+            const int* fo = a.foff + 3 * slot;
+            const double* fb = a.fxbase + 3 * slot;
+            if (part == 0) {
+                v.x = m;
+            } else if (part == 10) {
+                v.x = (double)(x + fo[0]) * a.dx + fb[0] - kInx * a.dx;
+                v.y = (double)(y + fo[1]) * a.dx + fb[1] - kInx * a.dx;
+            } else if (part == 11) {
+                v.x = (double)(z + fo[2]) * a.dx + fb[2] - kInx * a.dx;
+                v.y = m;
+            }
+        }
+        dst[ch] = v;
+    }
 }
 
 struct LaneGeom {
@@ -1633,7 +1655,7 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
     sa.list = nullptr;
     sa.dx = dx;
     sa.nslots = 27;
-    hipLaunchKernelGGL(stage_records, dim3(27 * kCells / 256), dim3(256), 0, st, sa);
+    hipLaunchKernelGGL(stage_records, dim3(27 * 4), dim3(256), 0, st, sa);
     HIPC(c, hipGetLastError());
     const size_t nL = (size_t)kCells * 20, nC = (size_t)kCells * 3;
     double* L13 = c->d_L + 13 * nL;
@@ -1913,9 +1935,8 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         sa.list = p->d_stage_list;
         sa.dx = p->dx;
         sa.nslots = p->d_stage_list ? p->n_stage : p->nslots;
-        const int nthreads = sa.nslots * kCells;
-        if (nthreads > 0) {
-            hipLaunchKernelGGL(stage_records, dim3((nthreads + 255) / 256), dim3(256), 0, st, sa);
+        if (sa.nslots > 0) {
+            hipLaunchKernelGGL(stage_records, dim3(sa.nslots * 4), dim3(256), 0, st, sa);
             HIPC(c, hipGetLastError());
         }
     }
