@@ -193,6 +193,27 @@ def test_level_kernel_variants(ctx, variant, t, monkeypatch):
         assert normwise(Lc[tg], Lcr[tg]) <= TOL
 
 
+@pytest.mark.parametrize("theta", [0.5, 0.34])
+@pytest.mark.parametrize("variant", [8, 108, 208, 304, 404, 504])
+def test_level_edge_targets_missing_neighbours(theta, variant, monkeypatch):
+    """Targets on the faces, edges and corners of the domain, whose partner
+    coarse cells beyond it do not exist: every lane mapping must leave those
+    pairs out."""
+    monkeypatch.setenv("OCTOFMM_VARIANT", str(variant))
+    c = native.Context(theta=theta, device=0)
+    lv = workloads.uniform_level((3, 3, 3), ((0, 3),) * 3, seed=31)
+    print(f"edge targets theta {theta} variant {variant}", flush=True)
+    outs, stats = run_level(c, lv, _abi.RHO)
+    c.close()
+    assert stats["variant"] == variant
+    assert (lv.neighbors[lv.targets] < 0).any()
+    L, Lc = outs[0]
+    Lr, Lcr = oracle.level_interactions(lv, theta=theta, type_=_abi.RHO)
+    tg = lv.targets
+    assert normwise(L[tg], Lr[tg]) <= TOL
+    assert normwise(Lc[tg], Lcr[tg]) <= TOL
+
+
 def test_autotuned_plan_leaves_outputs_untouched_until_executed(ctx):
     lv = workloads.uniform_level((4, 4, 4), ((1, 3),) * 3, seed=21)
     dl = hip.LevelOnDevice(lv)
