@@ -748,16 +748,28 @@ __device__ void m2l_item_direct2(const LevelArgs& a, int slot, int flags, const 
         const int cb = ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
         return reinterpret_cast<const char*>(a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles);
     };
-    // 8 loads on the lower 48 lanes: chunk l of children 4h..4h+3 of group gi
+    // 8 loads on the lower 48 lanes: chunk l of children 4h..4h+3 of group gi.
+    // The 8 partner bases of an offset are formed once, all their scalar
+    // loads in flight together, and serve both halves.
     const unsigned loff = (unsigned)l * 16u;
-    auto issue = [&](int di, int h) {
+    const char* nbase[8];
+    auto bases = [&](int di) {
         const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
-        double2* buf = half[h];
 #pragma unroll
-        for (int gi = 0; gi < 8; ++gi) {
-            const char* b = group_base(gi, ax, ay, az) + h * 48 * 16;
-            const unsigned dst = (unsigned)__builtin_amdgcn_readfirstlane((int)lds_offset(buf + gi * kHalfGroup));
-            if (l < 48) lds_dma16(b, loff, dst);
+        for (int gi = 0; gi < 8; ++gi) nbase[gi] = group_base(gi, ax, ay, az);
+        // formed here, so the 8 neighbour-slot loads share one wait instead of
+        // each waiting next to its refill
+        asm volatile("" ::"s"(nbase[0]), "s"(nbase[1]), "s"(nbase[2]), "s"(nbase[3]), "s"(nbase[4]),
+                     "s"(nbase[5]), "s"(nbase[6]), "s"(nbase[7]));
+    };
+    auto issue = [&](int h) {
+        double2* buf = half[h];
+        if (l < 48) {
+#pragma unroll
+            for (int gi = 0; gi < 8; ++gi) {
+                const unsigned dst = (unsigned)__builtin_amdgcn_readfirstlane((int)lds_offset(buf + gi * kHalfGroup));
+                lds_dma16(nbase[gi] + h * 48 * 16, loff, dst);
+            }
         }
     };
     auto far_bits = [&](int di) -> unsigned {
@@ -771,10 +783,17 @@ __device__ void m2l_item_direct2(const LevelArgs& a, int slot, int flags, const 
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0)
     asm volatile("" ::: "memory");
-    issue(0, 0);
-    issue(0, 1);
-    for (int di = 0; di < nd; ++di) {
-        const unsigned fb = far_bits(di);
+    // only the offsets with a far pair are staged (delta 0 is all near / self)
+    const int nfd = tab->nfar_delta;
+    if (nfd > 0) {
+        bases(tab->far_delta[0]);
+        issue(0);
+        issue(1);
+    }
+    for (int q = 0; q < nfd; ++q) {
+        const unsigned fb = far_bits(tab->far_delta[q]);
+        const bool more = q + 1 < nfd;
+        if (more) bases(tab->far_delta[q + 1]);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             // this half's 8 loads are done; the other half's 8 may still fly
@@ -791,8 +810,8 @@ __device__ void m2l_item_direct2(const LevelArgs& a, int slot, int flags, const 
                 }
             }
             lds_wave_sync();   // every read of this half done before it is refilled
-            if (di + 1 < nd) {
-                issue(di + 1, h);
+            if (more) {
+                issue(h);
             } else {
                 __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): nothing more in flight to wait behind
                 __builtin_amdgcn_wave_barrier();
