@@ -637,14 +637,30 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     // uniform base plus the lane's 16-byte offset, and the LDS destination is
     // uniform too, so a refill costs no per-lane address arithmetic.
     const unsigned loff = (unsigned)l * 16u;
-    auto issue = [&](int di) {
+    // The 8 partner bases of an offset are formed ahead of its refill, their
+    // neighbour-slot loads in flight together (one wait, taken while the
+    // current image is still arriving, instead of one wait per group).
+    const char* nbase[8];
+    auto bases = [&](int di) {
         const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
 #pragma unroll
-        for (int gi = 0; gi < 8; ++gi) {
-            const char* b = reinterpret_cast<const char*>(group_base(gi, ax, ay, az));
-            __builtin_amdgcn_global_load_lds(b + loff, (lds_ptr)(stage + gi * kDirGroup), 16, 0, 0);
-            if (l < 32)
-                __builtin_amdgcn_global_load_lds(b + 1024 + loff, (lds_ptr)(stage + gi * kDirGroup + 64), 16, 0, 0);
+        for (int gi = 0; gi < 8; ++gi) nbase[gi] = reinterpret_cast<const char*>(group_base(gi, ax, ay, az));
+        asm volatile("" ::"s"(nbase[0]), "s"(nbase[1]), "s"(nbase[2]), "s"(nbase[3]), "s"(nbase[4]),
+                     "s"(nbase[5]), "s"(nbase[6]), "s"(nbase[7]));
+    };
+    // The full-wave loads of all groups come first and the half-wave loads
+    // follow in one branch of their own: with a full and a half load of one
+    // group on the two sides of a branch, the compiler may merge them into one
+    // load whose LDS base differs by lane and is then taken from lane 0, which
+    // sends the other lanes' chunks to the wrong place.
+    auto issue = [&]() {
+#pragma unroll
+        for (int gi = 0; gi < 8; ++gi)
+            __builtin_amdgcn_global_load_lds(nbase[gi] + loff, (lds_ptr)(stage + gi * kDirGroup), 16, 0, 0);
+        if (l < 32) {
+#pragma unroll
+            for (int gi = 0; gi < 8; ++gi)
+                __builtin_amdgcn_global_load_lds(nbase[gi] + 1024 + loff, (lds_ptr)(stage + gi * kDirGroup + 64), 16, 0, 0);
         }
     };
     auto far_bits = [&](int di) -> unsigned {
@@ -656,9 +672,13 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     const double2* group = stage + c * kDirGroup;
     // only the offsets with a far pair are staged (delta 0 is all near / self)
     const int nfd = tab->nfar_delta;
-    if (nfd > 0) issue(tab->far_delta[0]);
+    if (nfd > 0) {
+        bases(tab->far_delta[0]);
+        issue();
+    }
     for (int q = 0; q < nfd; ++q) {
         const unsigned fb = far_bits(tab->far_delta[q]);
+        if (q + 1 < nfd) bases(tab->far_delta[q + 1]);
         __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): this wave's image is in LDS
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -671,7 +691,7 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             }
         }
         lds_wave_sync();   // every read of the image done before it is refilled
-        if (q + 1 < nfd) issue(tab->far_delta[q + 1]);
+        if (q + 1 < nfd) issue();
     }
     if (flags & kItemNear) {
         for (int di = 0; di < nd; ++di) {
