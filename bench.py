@@ -53,6 +53,8 @@ def kernel_name(variant):
     mapping, waves = divmod(variant, 100)
     if mapping == 5:
         return f"level_kernel_split<RHO, {waves}>"
+    if mapping == 7:
+        return "level_kernel_dsplit<RHO>"
     if mapping >= 3:
         return f"level_kernel_direct<RHO, {waves}, {mapping}>"
     return f"level_kernel<RHO, {waves}, {mapping}>"

@@ -603,9 +603,14 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(c
     return (const __attribute__((address_space(4))) T*)p;
 }
 
-template <bool RHO>
+// SPLIT = 1: mapping 3, the wave walks every offset of its coarse block.
+// SPLIT = 4: mapping 7, the 4 waves of the workgroup share one coarse block
+// and wave sl walks offsets sl, sl + 4, ... (far and near); waves 1..3 then
+// hand their partial sums to wave 0 through their own images, and wave 0 adds
+// them in wave order, so the sum is the same on every run.
+template <bool RHO, int SPLIT = 1>
 __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind,
-    int w, double2* __restrict__ s_stage) {
+    int w, double2* __restrict__ s_stage, int sl = 0) {
     // this wave's image (wave-uniform address)
     double2* stage = s_stage + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * kDirWave;
     const LaneGeom g = lane_geom(w);
@@ -672,13 +677,14 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     const double2* group = stage + c * kDirGroup;
     // only the offsets with a far pair are staged (delta 0 is all near / self)
     const int nfd = tab->nfar_delta;
-    if (nfd > 0) {
-        bases(tab->far_delta[0]);
+    const int q0 = SPLIT == 1 ? 0 : __builtin_amdgcn_readfirstlane(sl);
+    if (q0 < nfd) {
+        bases(tab->far_delta[q0]);
         issue();
     }
-    for (int q = 0; q < nfd; ++q) {
+    for (int q = q0; q < nfd; q += SPLIT) {
         const unsigned fb = far_bits(tab->far_delta[q]);
-        if (q + 1 < nfd) bases(tab->far_delta[q + 1]);
+        if (q + SPLIT < nfd) bases(tab->far_delta[q + SPLIT]);
         __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): this wave's image is in LDS
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -691,10 +697,10 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             }
         }
         lds_wave_sync();   // every read of the image done before it is refilled
-        if (q + 1 < nfd) issue();
+        if (q + SPLIT < nfd) issue();
     }
     if (flags & kItemNear) {
-        for (int di = 0; di < nd; ++di) {
+        for (int di = q0; di < nd; di += SPLIT) {
             const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
             unsigned nb = (unsigned)(tab->nearbits[di] >> (8 * g.k)) & 0xffu;
             int ns, raw;
@@ -708,6 +714,28 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
                     m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, inv_mc0, mc, L, C);
                 }
             }
+        }
+    }
+    if constexpr (SPLIT > 1) {
+        // no refill is in flight and every read of the image is done, so the
+        // image can hold this wave's 64 x 23 partial sums (11.5 KiB)
+        double* part = reinterpret_cast<double*>(stage) + (threadIdx.x & 63) * 23;
+        if (q0 != 0) {
+#pragma unroll
+            for (int i = 0; i < 20; ++i) part[i] = L[i];
+            part[20] = C[0];
+            part[21] = C[1];
+            part[22] = C[2];
+        }
+        __syncthreads();
+        if (q0 != 0) return;
+        for (int o = 1; o < SPLIT; ++o) {
+            const double* po = reinterpret_cast<const double*>(s_stage + o * kDirWave) + (threadIdx.x & 63) * 23;
+#pragma unroll
+            for (int i = 0; i < 20; ++i) L[i] += po[i];
+            C[0] += po[20];
+            C[1] += po[21];
+            C[2] += po[22];
         }
     }
     finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
@@ -1163,6 +1191,26 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
         m2l_item_direct<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage);
 }
 
+// Mapping-7 kernel: one workgroup of 4 waves per 2x2x2 coarse block, the
+// coarse offsets split over the waves (mapping 3's staging in each).
+template <bool RHO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void level_kernel_dsplit(LevelArgs a) {
+    __shared__ int s_nbr[27];
+    __shared__ int s_kind[27];
+    __shared__ double2 s_stage[4 * kDirWave];
+    const int v = xcd_item(blockIdx.x, gridDim.x);
+    const int item = v >> 3, w = v & 7;
+    const int2 it = a.items[a.item0 + item];
+    if (threadIdx.x < 27) {
+        const int n = a.nbr[(size_t)it.x * 27 + threadIdx.x];
+        s_nbr[threadIdx.x] = n;
+        s_kind[threadIdx.x] = n >= 0 ? a.kind[n] : -1;
+    }
+    __syncthreads();
+    if (it.y & kItemLeaf) return;   // uniform per workgroup
+    m2l_item_direct<RHO, 4>(a, it.x, it.y, s_nbr, s_kind, w, s_stage, (int)(threadIdx.x >> 6));
+}
+
 template <bool RHO>
 void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, bool mixed = false) {
     const int waves = variant % 100;
@@ -1188,6 +1236,9 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
     case 404: hipLaunchKernelGGL((level_kernel_direct<RHO, 4, 4>), grid, block, 0, st, la); break;
     case 504:
         hipLaunchKernelGGL((level_kernel_split<RHO, 4>), dim3(nitems * 8), dim3(256), 0, st, la);
+        break;
+    case 704:
+        hipLaunchKernelGGL((level_kernel_dsplit<RHO>), dim3(nitems * 8), dim3(256), 0, st, la);
         break;
     case 402: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 4>), grid, block, 0, st, la); break;
     case 208: hipLaunchKernelGGL((level_kernel<RHO, 8, 2>), grid, block, 0, st, la); break;

@@ -85,7 +85,9 @@ struct Tables {
 //   mapping 5: the coarse offsets of each 2x2x2 coarse block split over 4
 //              waves (variant 504), for levels with few sub-grids.  A 2-way
 //              split was measured slower at every level size tried.
-constexpr int kVariants[14] = {8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504};
+//   mapping 7: mapping 3's staging with mapping 5's 4-way offset split
+//              (variant 704); the partial sums meet in the waves' images.
+constexpr int kVariants[15] = {8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704};
 
 // Builds the tables; returns false when theta is below the floor or the
 // stencil fails the reference's symmetry / width checks.
