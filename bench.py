@@ -54,7 +54,7 @@ def kernel_name(variant):
     if mapping == 5:
         return f"level_kernel_split<RHO, {waves}>"
     if mapping == 7:
-        return "level_kernel_dsplit<RHO>"
+        return f"level_kernel_dsplit<RHO, {waves}>"
     if mapping >= 3:
         return f"level_kernel_direct<RHO, {waves}, {mapping}>"
     return f"level_kernel<RHO, {waves}, {mapping}>"

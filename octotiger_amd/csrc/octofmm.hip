@@ -604,10 +604,11 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(c
 }
 
 // SPLIT = 1: mapping 3, the wave walks every offset of its coarse block.
-// SPLIT = 4: mapping 7, the 4 waves of the workgroup share one coarse block
-// and wave sl walks offsets sl, sl + 4, ... (far and near); waves 1..3 then
-// hand their partial sums to wave 0 through their own images, and wave 0 adds
-// them in wave order, so the sum is the same on every run.
+// SPLIT = 2 or 4: mapping 7, the SPLIT waves of the workgroup share one
+// coarse block and wave sl walks offsets sl, sl + SPLIT, ... (far and near);
+// the other waves then hand their partial sums to wave 0 through their own
+// images, and wave 0 adds them in wave order, so the sum is the same on every
+// run.
 template <bool RHO, int SPLIT = 1>
 __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind,
     int w, double2* __restrict__ s_stage, int sl = 0) {
@@ -1191,13 +1192,13 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
         m2l_item_direct<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage);
 }
 
-// Mapping-7 kernel: one workgroup of 4 waves per 2x2x2 coarse block, the
+// Mapping-7 kernel: one workgroup of SPLIT waves per 2x2x2 coarse block, the
 // coarse offsets split over the waves (mapping 3's staging in each).
-template <bool RHO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void level_kernel_dsplit(LevelArgs a) {
+template <bool RHO, int SPLIT>
+__global__ __launch_bounds__(64 * SPLIT) __attribute__((amdgpu_waves_per_eu(3))) void level_kernel_dsplit(LevelArgs a) {
     __shared__ int s_nbr[27];
     __shared__ int s_kind[27];
-    __shared__ double2 s_stage[4 * kDirWave];
+    __shared__ double2 s_stage[SPLIT * kDirWave];
     const int v = xcd_item(blockIdx.x, gridDim.x);
     const int item = v >> 3, w = v & 7;
     const int2 it = a.items[a.item0 + item];
@@ -1208,7 +1209,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
     }
     __syncthreads();
     if (it.y & kItemLeaf) return;   // uniform per workgroup
-    m2l_item_direct<RHO, 4>(a, it.x, it.y, s_nbr, s_kind, w, s_stage, (int)(threadIdx.x >> 6));
+    m2l_item_direct<RHO, SPLIT>(a, it.x, it.y, s_nbr, s_kind, w, s_stage, (int)(threadIdx.x >> 6));
 }
 
 template <bool RHO>
@@ -1238,7 +1239,10 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
         hipLaunchKernelGGL((level_kernel_split<RHO, 4>), dim3(nitems * 8), dim3(256), 0, st, la);
         break;
     case 704:
-        hipLaunchKernelGGL((level_kernel_dsplit<RHO>), dim3(nitems * 8), dim3(256), 0, st, la);
+        hipLaunchKernelGGL((level_kernel_dsplit<RHO, 4>), dim3(nitems * 8), dim3(256), 0, st, la);
+        break;
+    case 702:
+        hipLaunchKernelGGL((level_kernel_dsplit<RHO, 2>), dim3(nitems * 8), dim3(128), 0, st, la);
         break;
     case 402: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 4>), grid, block, 0, st, la); break;
     case 208: hipLaunchKernelGGL((level_kernel<RHO, 8, 2>), grid, block, 0, st, la); break;

@@ -85,9 +85,10 @@ struct Tables {
 //   mapping 5: the coarse offsets of each 2x2x2 coarse block split over 4
 //              waves (variant 504), for levels with few sub-grids.  A 2-way
 //              split was measured slower at every level size tried.
-//   mapping 7: mapping 3's staging with mapping 5's 4-way offset split
-//              (variant 704); the partial sums meet in the waves' images.
-constexpr int kVariants[15] = {8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704};
+//   mapping 7: mapping 3's staging with the coarse offsets of a block split
+//              over 4 or 2 waves (variants 704, 702); the partial sums meet in
+//              the waves' images.
+constexpr int kVariants[16] = {8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702};
 
 // Builds the tables; returns false when theta is below the floor or the
 // stencil fails the reference's symmetry / width checks.
