@@ -523,6 +523,24 @@ __device__ __forceinline__ void multipole_shift(const double* A, const double* X
             }
 }
 
+// multipole_shift for A = (m, 0, ..., 0), a cell that holds only its mass:
+// the terms of the general form that remain, each formed in the same order.
+__device__ __forceinline__ void monopole_shift(double m, const double* X, double* B) {
+    B[0] = m;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) B[1 + a] = m * X[a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = a; b < 3; ++b) B[tmap2(a, b)] = m * X[a] * X[b];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = a; b < 3; ++b)
+#pragma unroll
+            for (int c = b; c < 3; ++c) B[tmap3(a, b, c)] = m * X[a] * X[b] * X[c];
+}
+
 // taylor::operator<<= : B = local expansion A re-expanded at +X.
 __device__ __forceinline__ void local_shift(const double* A, const double* X, double* B) {
 #pragma unroll

@@ -1341,20 +1341,24 @@ __global__ __launch_bounds__(256) void m2m_kernel(M2MArgs a) {
     if (gid >= a.nnodes * kCells) return;     // whole 8-lane groups exit together
     const int n = gid >> 9, q = (gid >> 3) & 63, ci = gid & 7;
     const int ip = q >> 4, jp = (q >> 2) & 3, kp = q & 3;
-    const bool leaf = a.kind[n] == OCTOFMM_SLOT_LEAF;
+    const bool leaf = a.kind[n] == OCTOFMM_SLOT_LEAF;   // uniform per node, so per wave
     const int u = (2 * ip + (ci & 1)) * 64 + (2 * jp + ((ci >> 1) & 1)) * 8 + (2 * kp + ((ci >> 2) & 1));
-    double* X = a.com + ((size_t)n * kCells + u) * 3;
+    const double* xb = a.xbase + 3 * n;
     if (leaf && a.rho) {
-        const double* xb = a.xbase + 3 * n;
-        X[0] = cell_centre(xb[0], u >> 6, a.dx);
-        X[1] = cell_centre(xb[1], (u >> 3) & 7, a.dx);
-        X[2] = cell_centre(xb[2], u & 7, a.dx);
+        // the leaf cells' centres, consecutive lanes on consecutive cells so
+        // the stores coalesce
+        const int v = gid & (kCells - 1);
+        double* Xv = a.com + ((size_t)n * kCells + v) * 3;
+        Xv[0] = cell_centre(xb[0], v >> 6, a.dx);
+        Xv[1] = cell_centre(xb[1], (v >> 3) & 7, a.dx);
+        Xv[2] = cell_centre(xb[2], v & 7, a.dx);
     }
     const int pn = a.parent[n];
     if (pn < 0) return;                        // uniform per node
     const int pc = parent_cell(a.octant[n], ip, jp, kp);
     double* Yp = a.pcom + ((size_t)pn * kCells + pc) * 3;
     double A[20];
+    double x0, x1, x2;
     if (leaf) {
         A[0] = a.mon[(size_t)n * kCells + u];
 #pragma unroll
@@ -1368,7 +1372,17 @@ __global__ __launch_bounds__(256) void m2m_kernel(M2MArgs a) {
             A[2 * j + 1] = v.y;
         }
     }
-    const double x0 = X[0], x1 = X[1], x2 = X[2];
+    if (leaf && a.rho) {
+        // the centre stored above for cell u, formed again rather than read back
+        x0 = cell_centre(xb[0], u >> 6, a.dx);
+        x1 = cell_centre(xb[1], (u >> 3) & 7, a.dx);
+        x2 = cell_centre(xb[2], u & 7, a.dx);
+    } else {
+        const double* X = a.com + ((size_t)n * kCells + u) * 3;
+        x0 = X[0];
+        x1 = X[1];
+        x2 = X[2];
+    }
     double Y[3];
     if (a.rho) {
         const double mtot = sum8(A[0]);
@@ -1387,7 +1401,10 @@ __global__ __launch_bounds__(256) void m2m_kernel(M2MArgs a) {
     }
     const double dX[3] = {x0 - Y[0], x1 - Y[1], x2 - Y[2]};
     double B[20];
-    multipole_shift(A, dX, B);
+    if (leaf)
+        monopole_shift(A[0], dX, B);
+    else
+        multipole_shift(A, dX, B);
     double* Mo = a.pM + ((size_t)pn * kCells + pc) * 20;
 #pragma unroll
     for (int j = 0; j < 20; ++j) {
