@@ -1070,21 +1070,65 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
         __syncthreads();
         for (int i = threadIdx.x; i < kCells * 7; i += 512) acc[i] = 0.0;
         __syncthreads();
+        if constexpr (CUBE) {
+            // Opposite directions r and 26 - r reach disjoint target cells (the
+            // stencil reaches 2 cells past the sub-grid, less than half of it;
+            // the host checks this), so they share one phase: two slabs, one
+            // target list, half the barriers, and the few targets of an edge
+            // or corner direction fill more of the workgroup.  Each target
+            // still receives its directions' sums in a fixed order.
+            double* slab2 = slab + kP2mSlab * kRecDoubles;
+            for (int r = 0; r < 13; ++r) {
+                const int ra = r, rb = 26 - r;
+                const bool on_a = s_nbr[ra] >= 0 && s_kind[ra] == OCTOFMM_SLOT_NONLEAF;
+                const bool on_b = s_nbr[rb] >= 0 && s_kind[rb] == OCTOFMM_SLOT_NONLEAF;
+                if (!on_a && !on_b) continue;   // uniform
+                const int nta = on_a ? a.p2m_dir[2 * ra] : 0, ntb = on_b ? a.p2m_dir[2 * rb] : 0;
+                const int sna = on_a ? a.p2m_sdir[2 * ra] : 0, snb = on_b ? a.p2m_sdir[2 * rb] : 0;
+                const int soa = a.p2m_sdir[2 * ra + 1], sob = a.p2m_sdir[2 * rb + 1];
+                const double2* srca = reinterpret_cast<const double2*>(
+                    a.rec + (size_t)(on_a ? s_nbr[ra] : 0) * kCells * kRecDoubles);
+                const double2* srcb = reinterpret_cast<const double2*>(
+                    a.rec + (size_t)(on_b ? s_nbr[rb] : 0) * kCells * kRecDoubles);
+                double2* dsta = reinterpret_cast<double2*>(slab);
+                double2* dstb = reinterpret_cast<double2*>(slab2);
+                for (int c2 = threadIdx.x; c2 < (sna + snb) * 12; c2 += 512) {
+                    if (c2 < sna * 12)
+                        dsta[c2] = srca[(size_t)a.p2m_slab[soa + c2 / 12] * 12 + c2 % 12];
+                    else {
+                        const int c3 = c2 - sna * 12;
+                        dstb[c3] = srcb[(size_t)a.p2m_slab[sob + c3 / 12] * 12 + c3 % 12];
+                    }
+                }
+                __syncthreads();
+                const int t0a = a.p2m_dir[2 * ra + 1], t0b = a.p2m_dir[2 * rb + 1];
+                for (int q = threadIdx.x; q < (nta + ntb) * 8; q += 512) {
+                    const int sub = q & 7, ti = q >> 3;   // 8-lane groups never straddle the two lists
+                    const bool second = ti >= nta;
+                    const int4 te = a.p2m_tgt[second ? t0b + (ti - nta) : t0a + ti];
+                    const double* pb = second ? slab2 : slab;
+                    const double* xc = a.com + ((size_t)slot * kCells + te.x) * 3;
+                    const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
+                    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                    for (int e = sub; e < te.z; e += 8)
+                        p2m_pair<RHO>(pb + (size_t)a.p2m_spart[te.y + e] * kRecDoubles, X0, X1, X2, v, v + 4);
+#pragma unroll
+                    for (int i = 0; i < 7; ++i) v[i] = sum8(v[i]);
+                    if (sub == 0) {
+                        double* o = acc + te.x * 7;
+#pragma unroll
+                        for (int i = 0; i < 7; ++i) o[i] += v[i];
+                    }
+                }
+                __syncthreads();
+            }
+        } else
         for (int r = 0; r < 27; ++r) {
             const int ns = s_nbr[r];
             if (r == 13 || ns < 0 || s_kind[r] != OCTOFMM_SLOT_NONLEAF) continue;
             const int nt = a.p2m_dir[2 * r], t0 = a.p2m_dir[2 * r + 1];
-            const double* base = a.rec + (size_t)ns * kCells * kRecDoubles;
-            if constexpr (CUBE) {
-                const int sn = a.p2m_sdir[2 * r], so = a.p2m_sdir[2 * r + 1];
-                const double2* src = reinterpret_cast<const double2*>(base);
-                double2* dst = reinterpret_cast<double2*>(slab);
-                for (int c2 = threadIdx.x; c2 < sn * 12; c2 += 512)
-                    dst[c2] = src[(size_t)a.p2m_slab[so + c2 / 12] * 12 + c2 % 12];
-                __syncthreads();
-            }
-            const double* pbase = CUBE ? slab : base;
-            const int32_t* pidx = CUBE ? a.p2m_spart : a.p2m_part;
+            const double* pbase = a.rec + (size_t)ns * kCells * kRecDoubles;
+            const int32_t* pidx = a.p2m_part;
             for (int q = threadIdx.x; q < nt * 8; q += 512) {
                 const int sub = q & 7;
                 const int4 te = a.p2m_tgt[t0 + (q >> 3)];
@@ -1125,12 +1169,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
 }
 
 template <bool RHO>
-void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, const Tables& tab, int slab_max) {
+void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, const Tables& tab, int slab_max, bool opp_disjoint) {
     const int W = kInx + 2 * tab.p2p_halo;
-    const bool cube = tab.p2p_cube && slab_max <= kP2mSlab;
-    // window, p2m sums (512 x 7) with the p2m slab on the cube path, and the
+    const bool cube = tab.p2p_cube && slab_max <= kP2mSlab && opp_disjoint;
+    // window, p2m sums (512 x 7) with two p2m slabs on the cube path, and the
     // staged L rows of half a sub-grid (256 x 20)
-    const size_t p2m = (size_t)kCells * 7 + (cube ? (size_t)kP2mSlab * kRecDoubles : 0);
+    const size_t p2m = (size_t)kCells * 7 + (cube ? (size_t)2 * kP2mSlab * kRecDoubles : 0);
     const size_t lds = sizeof(double) * std::max<size_t>(std::max<size_t>((size_t)W * W * W, p2m), (size_t)256 * 20);
     if (cube)
         hipLaunchKernelGGL((leaf_kernel<RHO, true>), dim3(nitems), dim3(512), lds, st, la);
@@ -1452,6 +1496,7 @@ struct octofmm_context {
     int32_t* d_p2m_slab = nullptr;
     int32_t* d_p2m_spart = nullptr;
     int32_t p2m_slab_max = 0;      // largest per-direction slab (records)
+    bool p2m_opp_disjoint = false; // opposite directions reach disjoint target cells
 };
 
 struct octofmm_plan {
@@ -1667,7 +1712,7 @@ octofmm_status fill_workspace(octofmm_context* c, const octofmm_subgrid* in, boo
 // (coarse offset, child) order.
 void build_p2m_lists(const Tables& T, std::vector<int32_t>& dir, std::vector<int4>& tgt,
     std::vector<int32_t>& part, std::vector<int32_t>& sdir, std::vector<int32_t>& slab,
-    std::vector<int32_t>& spart, int& slab_max) {
+    std::vector<int32_t>& spart, int& slab_max, bool& opp_disjoint) {
     std::vector<int32_t> lists[27][kCells];
     for (int t = 0; t < kCells; ++t) {
         const int x = t >> 6, y = (t >> 3) & 7, z = t & 7;
@@ -1721,6 +1766,12 @@ void build_p2m_lists(const Tables& T, std::vector<int32_t>& dir, std::vector<int
         for (int q = dir[2 * r + 1]; q < dir[2 * r + 1] + dir[2 * r]; ++q)
             for (int e = 0; e < tgt[q].z; ++e) spart[tgt[q].y + e] = pos[part[tgt[q].y + e]];
     }
+    // the leaf kernel's theta = 0.5 path runs directions r and 26 - r in one
+    // phase, which needs their target cells disjoint
+    opp_disjoint = true;
+    for (int r = 0; r < 13; ++r)
+        for (int t = 0; t < kCells; ++t)
+            if (!lists[r][t].empty() && !lists[26 - r][t].empty()) opp_disjoint = false;
 }
 
 // Uploads the staged 27-slot level and its tables.
@@ -1825,9 +1876,9 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         la.neg_inv_dx = -1.0 / dx;
         if (item_flags & kItemLeaf) {
             if (rho)
-                launch_leaf<true>(1, st, la, c->tab, c->p2m_slab_max);
+                launch_leaf<true>(1, st, la, c->tab, c->p2m_slab_max, c->p2m_opp_disjoint);
             else
-                launch_leaf<false>(1, st, la, c->tab, c->p2m_slab_max);
+                launch_leaf<false>(1, st, la, c->tab, c->p2m_slab_max, c->p2m_opp_disjoint);
         } else if (rho) {
             launch_level<true>(8, 1, st, la);
         } else {
@@ -1888,7 +1939,7 @@ octofmm_status octofmm_create(const octofmm_config* cfg, octofmm_context** out) 
     }
     std::vector<int32_t> pdir, ppart, psdir, pslab, pspart;
     std::vector<int4> ptgt;
-    build_p2m_lists(c->tab, pdir, ptgt, ppart, psdir, pslab, pspart, c->p2m_slab_max);
+    build_p2m_lists(c->tab, pdir, ptgt, ppart, psdir, pslab, pspart, c->p2m_slab_max, c->p2m_opp_disjoint);
     if (hipMalloc((void**)&c->d_p2m_dir, sizeof(int32_t) * pdir.size()) != hipSuccess ||
         hipMalloc((void**)&c->d_p2m_tgt, sizeof(int4) * std::max<size_t>(1, ptgt.size())) != hipSuccess ||
         hipMalloc((void**)&c->d_p2m_part, sizeof(int32_t) * std::max<size_t>(1, ppart.size())) != hipSuccess ||
@@ -2115,9 +2166,9 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
             la.nitems = p->nitems - p->n_m2l;
             la.item0 = p->n_m2l;
             if (rho)
-                launch_leaf<true>(la.nitems, st, la, c->tab, c->p2m_slab_max);
+                launch_leaf<true>(la.nitems, st, la, c->tab, c->p2m_slab_max, c->p2m_opp_disjoint);
             else
-                launch_leaf<false>(la.nitems, st, la, c->tab, c->p2m_slab_max);
+                launch_leaf<false>(la.nitems, st, la, c->tab, c->p2m_slab_max, c->p2m_opp_disjoint);
             HIPC(c, hipGetLastError());
         }
     }
