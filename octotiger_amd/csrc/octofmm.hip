@@ -604,7 +604,7 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(c
 }
 
 // SPLIT = 1: mapping 3, the wave walks every offset of its coarse block.
-// SPLIT = 2 or 4: mapping 7, the SPLIT waves of the workgroup share one
+// SPLIT = 2, 4 or 8: mapping 7, the SPLIT waves of the workgroup share one
 // coarse block and wave sl walks offsets sl, sl + SPLIT, ... (far and near);
 // the other waves then hand their partial sums to wave 0 through their own
 // images, and wave 0 adds them in wave order, so the sum is the same on every
@@ -1287,6 +1287,9 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
         break;
     case 702:
         hipLaunchKernelGGL((level_kernel_dsplit<RHO, 2>), dim3(nitems * 8), dim3(128), 0, st, la);
+        break;
+    case 708:
+        hipLaunchKernelGGL((level_kernel_dsplit<RHO, 8>), dim3(nitems * 8), dim3(512), 0, st, la);
         break;
     case 402: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 4>), grid, block, 0, st, la); break;
     case 208: hipLaunchKernelGGL((level_kernel<RHO, 8, 2>), grid, block, 0, st, la); break;

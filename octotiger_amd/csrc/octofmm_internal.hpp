@@ -86,9 +86,10 @@ struct Tables {
 //              waves (variant 504), for levels with few sub-grids.  A 2-way
 //              split was measured slower at every level size tried.
 //   mapping 7: mapping 3's staging with the coarse offsets of a block split
-//              over 4 or 2 waves (variants 704, 702); the partial sums meet in
-//              the waves' images.
-constexpr int kVariants[16] = {8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702};
+//              over 4, 2 or 8 waves (variants 704, 702, 708); the partial sums
+//              meet in the waves' images.  The 8-wave split (one workgroup per
+//              CU by its LDS) is for levels with very few sub-grids.
+constexpr int kVariants[17] = {8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702, 708};
 
 // Builds the tables; returns false when theta is below the floor or the
 // stencil fails the reference's symmetry / width checks.

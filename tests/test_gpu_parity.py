@@ -133,7 +133,7 @@ def test_level_mixed_kinds(ctx, t, graph):
 
 
 @pytest.mark.parametrize("theta", [0.34, 0.45])
-@pytest.mark.parametrize("variant", [8, 108, 208, 304, 404, 504, 704])
+@pytest.mark.parametrize("variant", [8, 108, 208, 304, 404, 504, 704, 708])
 def test_level_mixed_kinds_other_theta(theta, variant, monkeypatch):
     """Wider stencils (theta 0.34: coarse offsets up to 2, p2p window 18^3)
     under every lane mapping, the staged ones included."""
@@ -175,7 +175,7 @@ def test_level_config2_full_size(ctx):
 
 
 @pytest.mark.parametrize("t", [_abi.RHO, _abi.DRHODT])
-@pytest.mark.parametrize("variant", [8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702])
+@pytest.mark.parametrize("variant", [8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702, 708])
 def test_level_kernel_variants(ctx, variant, t, monkeypatch):
     """Every lane mapping and workgroup shape of the level kernel gives the
     oracle's result (mixed level: M2L far + near passes, p2p, p2m), for both
@@ -194,7 +194,7 @@ def test_level_kernel_variants(ctx, variant, t, monkeypatch):
 
 
 @pytest.mark.parametrize("theta", [0.5, 0.34])
-@pytest.mark.parametrize("variant", [8, 108, 208, 304, 404, 504, 704])
+@pytest.mark.parametrize("variant", [8, 108, 208, 304, 404, 504, 704, 708])
 def test_level_edge_targets_missing_neighbours(theta, variant, monkeypatch):
     """Targets on the faces, edges and corners of the domain, whose partner
     coarse cells beyond it do not exist: every lane mapping must leave those
@@ -218,7 +218,7 @@ def test_autotuned_plan_leaves_outputs_untouched_until_executed(ctx):
     lv = workloads.uniform_level((4, 4, 4), ((1, 3),) * 3, seed=21)
     dl = hip.LevelOnDevice(lv)
     plan = dl.plan(ctx, type_=_abi.RHO, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
-    assert plan.stats()["variant"] in (8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702)
+    assert plan.stats()["variant"] in (8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702, 708)
     assert not dl.L.numpy().any() and not dl.L_c.numpy().any()
     stream = hip.Stream()
     plan.execute(stream.handle)
