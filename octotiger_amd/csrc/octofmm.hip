@@ -1330,14 +1330,25 @@ __global__ __launch_bounds__(512) void root_kernel(RootArgs a) {
     o[22] = C[2];
 }
 
-// One thread per output value (cell, component); the 64 chunks are summed in
-// chunk order, so the result does not depend on scheduling.
+// One thread per output value (cell, component); the chunks are summed in
+// chunk order, so the result does not depend on scheduling.  Sixteen chunks'
+// loads are issued before their adds, so the thread waits on one round trip
+// per sixteen chunks rather than one per chunk.
 __global__ __launch_bounds__(256) void root_reduce(RootReduceArgs a) {
     const int gid = blockIdx.x * 256 + threadIdx.x;
     if (gid >= kCells * 23) return;
     const int t = gid / 23, c = gid - 23 * (gid / 23);
+    const double* src = a.partial + (size_t)t * kRecDoubles + c;
+    const size_t stride = (size_t)kCells * kRecDoubles;
     double s = 0.0;
-    for (int q = 0; q < a.nchunks; ++q) s += a.partial[((size_t)q * kCells + t) * kRecDoubles + c];
+    for (int q0 = 0; q0 < a.nchunks; q0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = q0 + i < a.nchunks ? src[(size_t)(q0 + i) * stride] : 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (q0 + i < a.nchunks) s += v[i];
+    }
     if (c < 20)
         a.L[(size_t)t * 20 + c] = a.overwrite ? s : a.L[(size_t)t * 20 + c] + s;
     else if (a.rho)
