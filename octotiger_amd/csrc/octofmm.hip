@@ -1110,8 +1110,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
                     const double* xc = a.com + ((size_t)slot * kCells + te.x) * 3;
                     const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
                     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                    for (int e = sub; e < te.z; e += 8)
-                        p2m_pair<RHO>(pb + (size_t)a.p2m_spart[te.y + e] * kRecDoubles, X0, X1, X2, v, v + 4);
+                    // the next partner's index is requested one pair ahead,
+                    // so its round trip overlaps the current pair
+                    const int32_t* ip = a.p2m_spart + te.y;
+                    int nxt = sub < te.z ? ip[sub] : 0;
+                    for (int e = sub; e < te.z; e += 8) {
+                        const int cur = nxt;
+                        if (e + 8 < te.z) nxt = ip[e + 8];
+                        p2m_pair<RHO>(pb + (size_t)cur * kRecDoubles, X0, X1, X2, v, v + 4);
+                    }
 #pragma unroll
                     for (int i = 0; i < 7; ++i) v[i] = sum8(v[i]);
                     if (sub == 0) {
