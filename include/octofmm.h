@@ -306,8 +306,8 @@ octofmm_status octofmm_tree_plan_destroy(octofmm_tree_plan* plan);
 
 /* ---------------------------------------------------------------------------
  * Device FMA rate (diagnostic).  No counterpart in the reference: it measures
- * the dense vector FMA rate of the context's device, the peak the interaction
- * kernels are judged against.  Every lane runs 8 independent FMA chains
+ * the dense vector FMA rate of the context's device, the practical peak to
+ * set the interaction kernels' rates against.  Every lane runs 8 independent FMA chains
  * (precision OCTOFMM_PRECISION_FP64: v_fma_f64; OCTOFMM_PRECISION_MIXED: the
  * FP32 form, unpacked as all device code here is built) over enough
  * workgroups to fill every CU; *tflops = 2 x FMAs / time of the timed launch,
