@@ -641,7 +641,7 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     // Group c (8 records = 96 chunks) is loaded by one full instruction
     // (chunks 0..63) and one on the lower 32 lanes (chunks 64..95): both are a
     // uniform base plus the lane's 16-byte offset, and the LDS destination is
-    // uniform too, so a refill costs no per-lane address arithmetic.
+    // uniform too, so a refill costs one address add per load instruction.
     const unsigned loff = (unsigned)l * 16u;
     // The 8 partner bases of an offset are formed ahead of its refill, their
     // neighbour-slot loads in flight together (one wait, taken while the
