@@ -1,22 +1,26 @@
 """Benchmark of the FMM gravity interaction pass on one MI355X.
 
 One step = one pass of the level interaction path over the batched uniform
-level of 512 non-leaf 8^3 sub-grids (80^3 cells, theta = 0.5, RHO with the
-angular-momentum correction): AoS -> partner-record staging plus the
-interaction kernel, replayed as one captured hipGraph, kernel variant chosen
-by the plan's autotuner.  Inputs are resident in HBM before timing starts.
+level of 4096 non-leaf 8^3 sub-grids (a 16^3 block inside a one-slot halo
+shell: 18^3 slots, 144^3 cells, theta = 0.5, RHO with the angular-momentum
+correction): AoS -> partner-record staging plus the interaction kernel,
+replayed as one captured hipGraph, kernel variant chosen by the plan's
+autotuner.  Inputs are resident in HBM before timing starts.
 
 Prints one JSON line: cell-interactions/s (M2L pairs per second) of the FP64
 pass, the roofline of the dominant kernel (FP64 VALU), a CPU baseline (the
-reference's own interaction templates -- or the oracle when the reference
-tree was not available at build time -- on the host cores, over the same
-level), and the mixed-precision form on the same level and on the
-4096-sub-grid level with its error against the FP64 path.
+reference's own interaction templates on host-width vectors, -O3, walked as
+the Kokkos host kernel walks them, on the host cores over a sample of the same
+level), the mixed-precision form on the same level with its errors against
+the FP64 oracle, and the production step (DRHODT then RHO, as
+node_server::compute_fmm is called each step).
 
-It also reports the mixed-kind 2048-sub-grid level and the full adaptive-tree
-solve of a polytrope (10,697 sub-grids, M2M + interactions + L2L + forces).
+Extra keys: the 512-sub-grid level, the mixed-kind 2048-sub-grid level and
+the full adaptive-tree solve of a polytrope (10,697 sub-grids, M2M +
+interactions + L2L + forces), each with its production step.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-c3] [--no-c4] [--no-c5] [--cpu-repeats R]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-c2] [--no-c3] [--no-c5]
+                    [--cpu-repeats R] [--no-check]
 """
 import argparse
 import json
@@ -30,44 +34,49 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-interactions/s FP64 vs mixed-prec, % VALU peak, rel. force err vs FP64"
 FP64_VALU_PEAK_TFLOPS = 78.6     # MI355X FP64 vector, dense
 FLOPS_PER_PAIR = {"rho": 407, "drhodt": 273}
+CPU_THREADS_MAX = 16             # host threads one GPU's share of the box gives a process
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--no-c4", action="store_true", help="skip the 4096-sub-grid FP64/mixed comparison")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 512-sub-grid level")
     ap.add_argument("--no-c3", action="store_true", help="skip the mixed-kind 2048-sub-grid level")
     ap.add_argument("--no-c5", action="store_true", help="skip the full adaptive-tree solve")
-    ap.add_argument("--cpu-repeats", type=int, default=3, help="passes of the CPU baseline over the level (0 = skip)")
-    ap.add_argument("--no-check", action="store_true", help="skip the post-run parity spot check")
+    ap.add_argument("--cpu-repeats", type=int, default=2, help="passes of the CPU baseline over its sample (0 = skip)")
+    ap.add_argument("--no-check", action="store_true", help="skip the oracle comparison of the headline level")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run 3 device-to-device copies of this many MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
-def kernel_name(variant):
+def kernel_name(variant, rho=True):
     """Device kernel the plan's interaction launch runs for a variant
     (lane mapping x 100 + waves per workgroup)."""
+    t = "RHO" if rho else "DRHODT"
     mapping, waves = divmod(variant, 100)
     if mapping == 5:
-        return f"level_kernel_split<RHO, {waves}>"
+        return f"level_kernel_split<{t}, {waves}>"
     if mapping == 7:
-        return f"level_kernel_dsplit<RHO, {waves}>"
+        return f"level_kernel_dsplit<{t}, {waves}>"
     if mapping >= 3:
-        return f"level_kernel_direct<RHO, {waves}, {mapping}>"
-    return f"level_kernel<RHO, {waves}, {mapping}>"
+        return f"level_kernel_direct<{t}, {waves}, {mapping}>"
+    return f"level_kernel<{t}, {waves}, {mapping}>"
 
 
-def time_plan(plan, stream, steps, warmup, hip):
+def time_plans(plans, stream, steps, warmup, hip):
+    """Milliseconds per step of the plans executed back to back on one stream."""
     for _ in range(warmup):
-        plan.execute(stream.handle)
+        for p in plans:
+            p.execute(stream.handle)
     stream.synchronize()
     e0, e1 = hip.Event(), hip.Event()
     e0.record(stream)
     for _ in range(steps):
-        plan.execute(stream.handle)
+        for p in plans:
+            p.execute(stream.handle)
     e1.record(stream)
     e1.synchronize()
     ms = e0.elapsed_ms(e1) / steps
@@ -83,91 +92,138 @@ def one_pass(plan, dl, stream):
     return dl.L.numpy(), dl.L_c.numpy()
 
 
-def mixed_errors(lv, Lm, Lcm, Lr, Lcr):
-    """Error of the mixed-precision pass against the FP64 pass over all
-    target cells: per component normwise (max cell error / max |ref|),
-    elementwise relative, and total force / torque."""
+def errors_vs(lv, L, Lc, Lr, Lcr):
+    """Errors of a pass against a reference over all target cells: per
+    component normwise (max cell error / max |ref|), elementwise relative, and
+    the total force / torque relative to the reference's totals."""
     import numpy as np
 
-    from metrics import force_torque_error
+    from metrics import force_torque_errors, normwise
     tg = lv.targets
-    a = np.concatenate([Lm[tg], Lcm[tg]], axis=-1).reshape(-1, 23)
+    a = np.concatenate([L[tg], Lc[tg]], axis=-1).reshape(-1, 23)
     b = np.concatenate([Lr[tg], Lcr[tg]], axis=-1).reshape(-1, 23)
     scale = np.abs(b).max(axis=0)
-    normwise = np.abs(a - b).max(axis=0) / np.where(scale > 0, scale, 1.0)
+    nw = np.abs(a - b).max(axis=0) / np.where(scale > 0, scale, 1.0)
     elem = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
-    ef, et = force_torque_error(Lm[tg], Lcm[tg], Lr[tg], Lcr[tg], lv.M[tg, :, 0], lv.com[tg])
+    ft = force_torque_errors(L[tg], Lc[tg], Lr[tg], Lcr[tg], lv.M[tg, :, 0], lv.com[tg])
     return {
-        "max_normwise_rel_err": float(normwise.max()),
-        "normwise_rel_err_per_component": [float(x) for x in normwise],
+        "max_normwise_rel_err": float(nw.max()),
+        "normwise_rel_err_L": float(nw[:20].max()),
+        "normwise_rel_err_L_c": float(nw[20:].max()),
+        "normwise_rel_err_per_component": [float(x) for x in nw],
         "max_elementwise_rel_err": float(elem.max()),
         "median_elementwise_rel_err": float(np.median(elem)),
-        "force_rel_err": ef,
-        "torque_rel_err": et,
+        "force_rel_err": ft["force"],
+        "torque_rel_err": ft["torque"],
+        "force_rel_err_per_cell_sum": ft["force_per_cell_sum"],
+        "torque_rel_err_per_cell_sum": ft["torque_per_cell_sum"],
     }
 
 
-def run_level(ctx, lv, args, hip, _abi, with_mixed=True):
+def run_level(ctx, lv, args, hip, _abi, ref=None, with_mixed=True, with_drhodt=True):
+    """FP64 (and mixed) RHO passes of a level, the DRHODT pass, and the
+    production step (DRHODT then RHO); errors against ref = (L, L_c) of the
+    oracle when given."""
     dl = hip.LevelOnDevice(lv)
     stream = hip.Stream()
     out = {}
-    p64 = dl.plan(ctx, type_=_abi.RHO, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
+    flags = _abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE
+    iters = max(3, min(args.steps, 20))
+    t0 = time.perf_counter()
+    p64 = dl.plan(ctx, type_=_abi.RHO, flags=flags)
+    create_s = time.perf_counter() - t0
     st64 = p64.stats()
-    ms64 = time_plan(p64, stream, args.steps, args.warmup, hip)
-    stage_ms = p64.time_stages(stream.handle, iters=max(3, min(args.steps, 20)))
+    ms64 = time_plans([p64], stream, args.steps, args.warmup, hip)
+    stage_ms = p64.time_stages(stream.handle, iters=iters)
     L64, Lc64 = one_pass(p64, dl, stream)
-    out["fp64"] = {"ms_per_step": ms64, "stage_ms": stage_ms, "stats": st64}
+    out["fp64"] = {"ms_per_step": ms64, "stage_ms": stage_ms, "stats": st64, "plan_create_s": create_s}
+    if ref is not None:
+        out["fp64"]["errors_vs_oracle"] = errors_vs(lv, L64, Lc64, *ref)
+    plans = {"fp64": p64}
     if with_mixed:
-        pm = dl.plan(ctx, type_=_abi.RHO, precision=_abi.PRECISION_MIXED, mixed_far_d2=4,
-                     flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
+        pm = dl.plan(ctx, type_=_abi.RHO, precision=_abi.PRECISION_MIXED, mixed_far_d2=4, flags=flags)
         stm = pm.stats()
-        msm = time_plan(pm, stream, args.steps, args.warmup, hip)
-        stage_m = pm.time_stages(stream.handle, iters=max(3, min(args.steps, 20)))
+        msm = time_plans([pm], stream, args.steps, args.warmup, hip)
+        stage_m = pm.time_stages(stream.handle, iters=iters)
         Lm, Lcm = one_pass(pm, dl, stream)
         out["mixed"] = {"ms_per_step": msm, "stage_ms": stage_m, "stats": stm,
-                        "errors": mixed_errors(lv, Lm, Lcm, L64, Lc64)}
-        pm.close()
-    p64.close()
+                        "errors_vs_fp64_pass": errors_vs(lv, Lm, Lcm, L64, Lc64)}
+        if ref is not None:
+            out["mixed"]["errors_vs_oracle"] = errors_vs(lv, Lm, Lcm, *ref)
+        plans["mixed"] = pm
+    if with_drhodt:
+        # node_server::compute_fmm(DRHODT) then compute_fmm(RHO) every step
+        # (src/node_server_actions_3.cpp:591-592)
+        step = {}
+        for name, prec in (("fp64", _abi.PRECISION_FP64), ("mixed", _abi.PRECISION_MIXED)):
+            if name not in plans:
+                continue
+            pd = dl.plan(ctx, type_=_abi.DRHODT, precision=prec, mixed_far_d2=4, flags=flags)
+            std = pd.stats()
+            msd = time_plans([pd], stream, args.steps, args.warmup, hip)
+            ms2 = time_plans([pd, plans[name]], stream, args.steps, args.warmup, hip)
+            pairs = sum(int(sx[k]) for sx in (std, plans[name].stats())
+                        for k in ("far_pairs", "near_pairs", "p2p_pairs", "p2m_pairs"))
+            step[name] = {"drhodt_ms": msd, "drhodt_kernel_variant": int(std["variant"]),
+                          "drhodt_then_rho_ms": ms2, "cell_interactions_per_s": pairs / (ms2 * 1e-3),
+                          "flops": int(std["flops"] + plans[name].stats()["flops"])}
+            pd.close()
+        if "mixed" in step:
+            step["mixed"]["speedup_vs_fp64"] = step["fp64"]["drhodt_then_rho_ms"] / step["mixed"]["drhodt_then_rho_ms"]
+            step["mixed"]["drhodt_speedup_vs_fp64"] = step["fp64"]["drhodt_ms"] / step["mixed"]["drhodt_ms"]
+        out["production_step"] = step
+    for p in plans.values():
+        p.close()
     stream.destroy()
     dl.free()
-    return out, (L64, Lc64)
+    return out
 
 
 def run_tree(ctx, args, hip, _abi, np):
     """Full adaptive-tree solve (upward M2M, interactions of every level, root
     stencil on level 0, downward L2L and leaf force assembly) as one hipGraph,
-    FP64 and mixed precision."""
+    FP64 and mixed precision, RHO and the production step DRHODT + RHO."""
     from octotiger_amd import workloads
 
     t = workloads.config5()
     dev = hip.TreeOnDevice(t)
     stream = hip.Stream()
+    flags = _abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE
+    steps = max(5, args.steps // 3)
     res = {}
     for name, prec in (("fp64", _abi.PRECISION_FP64), ("mixed", _abi.PRECISION_MIXED)):
-        plan = dev.plan(ctx, precision=prec, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
-        steps = max(5, args.steps // 5)
-        ms = time_plan(plan, stream, steps, max(1, args.warmup), hip)
+        t0 = time.perf_counter()
+        plan = dev.plan(ctx, precision=prec, flags=flags)
+        create_s = time.perf_counter() - t0
+        ms = time_plans([plan], stream, steps, max(1, args.warmup), hip)
         phases = plan.time_phases(stream.handle, iters=3)
         plan.execute(stream.handle)
         stream.synchronize()
         g = np.concatenate([d["g"].numpy()[lv.kind == _abi.SLOT_LEAF].reshape(-1, 4)
                             for lv, d in zip(t.levels, dev.levels)])
+        # production step: DRHODT (reusing this RHO solve's centres of mass), then RHO
+        pd = dev.plan(ctx, type_=_abi.DRHODT, precision=prec, flags=flags)
+        msd = time_plans([pd], stream, steps, 1, hip)
+        ms2 = time_plans([pd, plan], stream, steps, 1, hip)
+        pd.close()
         res[name] = {"ms_per_solve": ms, "phases_ms": {"upward_m2m": phases[0],
                                                        "interactions_l2l_forces": phases[1]},
-                     "stats": plan.stats(), "g": g}
+                     "stats": plan.stats(), "g": g, "plan_create_s": create_s,
+                     "production_step": {"drhodt_ms": msd, "drhodt_then_rho_ms": ms2}}
         plan.close()
     dev.free()
     # Octo-Tiger's own end-to-end regression value: the solid-sphere run
     # (test_problems/sphere/CMakeLists.txt:18) prints "pot 4.470774e-01 3.176879e+00"
     ts = workloads.solid_sphere_tree()
     dsp = hip.TreeOnDevice(ts)
-    plan = dsp.plan(ctx, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
-    ms_sphere = time_plan(plan, stream, max(5, args.steps // 5), max(1, args.warmup), hip)
+    plan = dsp.plan(ctx, flags=flags)
+    ms_sphere = time_plans([plan], stream, steps, max(1, args.warmup), hip)
     plan.execute(stream.handle)
     stream.synchronize()
     l1, l2 = workloads.sphere_pot_norms(ts, [d["g"].numpy()[..., 0] for d in dsp.levels])
     plan.close()
     dsp.free()
+    stream.destroy()
     got = f"pot {l1:e} {l2:e}"
     want = "pot {:e} {:e}".format(*workloads.SPHERE_POT_NORMS)
     sphere = {"workload": f"solid sphere, max_level 3, {ts.nnodes} sub-grids, nodes per level "
@@ -184,14 +240,91 @@ def run_tree(ctx, args, hip, _abi, np):
                     f"nodes per level {t.info['nodes_per_level']}, leaves per level {t.info['leaves_per_level']}",
         "fp64": {"ms_per_solve": res["fp64"]["ms_per_solve"], "phases_ms": res["fp64"]["phases_ms"],
                  "cell_interactions_per_s": pairs / (res["fp64"]["ms_per_solve"] * 1e-3),
-                 "interaction_valu_frac": st["flops"] / (inter * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS},
+                 "interaction_valu_frac": st["flops"] / (inter * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
+                 "plan_create_s": res["fp64"]["plan_create_s"],
+                 "production_step": res["fp64"]["production_step"]},
         "mixed": {"ms_per_solve": res["mixed"]["ms_per_solve"], "phases_ms": res["mixed"]["phases_ms"],
                   "speedup_vs_fp64": res["fp64"]["ms_per_solve"] / res["mixed"]["ms_per_solve"],
                   "fp32_far_pairs": int(res["mixed"]["stats"]["mixed_far_pairs"]),
-                  "leaf_force_rms_rel_err_vs_fp64": ef, "leaf_phi_max_rel_err_vs_fp64": ephi},
+                  "leaf_force_rms_rel_err_vs_fp64": ef, "leaf_phi_max_rel_err_vs_fp64": ephi,
+                  "plan_create_s": res["mixed"]["plan_create_s"],
+                  "production_step": res["mixed"]["production_step"]},
         "pairs": {k: int(st[k]) for k in ("far_pairs", "near_pairs", "p2p_pairs", "p2m_pairs", "root_pairs")},
         "nodes": int(st["nodes"]), "leaves": int(st["leaves"]),
         "solid_sphere_regression": sphere,
+    }
+
+
+def host_info():
+    """Logical CPUs, physical cores and model of the host, from /proc/cpuinfo."""
+    info = {"logical_cpus": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0))}
+    cores, model = set(), None
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("physical id"):
+                    phys = ln.split(":")[1].strip()
+                elif ln.startswith("core id"):
+                    core = ln.split(":")[1].strip()
+                elif ln.startswith("model name") and model is None:
+                    model = ln.split(":", 1)[1].strip()
+                elif not ln.strip():
+                    if core is not None:
+                        cores.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    info["physical_cores"] = len(cores) or None
+    info["model"] = model
+    return info
+
+
+def cpu_baseline(lv, pairs_per_subgrid, args):
+    """The reference's own interaction templates, vectorised across z-rows as
+    the Kokkos host kernel does, -O3, one sub-grid per OpenMP task, timed on
+    the host over a sample of the headline level."""
+    import numpy as np
+
+    import oracle
+    hi = host_info()
+    threads = max(1, min(CPU_THREADS_MAX, hi["usable_cpus"]))
+    simd = oracle.ref_simd_available()
+    if simd:
+        run = lambda tg, nt: oracle.level_interactions_simd(lv, targets=tg, nthreads=nt)  # noqa: E731
+        kind, width = "reference", oracle.host_simd_width()
+        how = (f"Octo-Tiger's own FP64 interaction templates (compute_kernel_templates.hpp) instantiated on "
+               f"{width}-wide double vectors, walked in the Kokkos host-kernel order (z-rows of 8 cells as "
+               f"vectors), g++ -O3 -march=x86-64-v{4 if width == 8 else 3}")
+    else:
+        use_ref = oracle.ref_available()
+        run = lambda tg, nt: oracle.level_interactions(lv, targets=tg, nthreads=nt, use_ref=use_ref)  # noqa: E731
+        kind = "reference" if use_ref else "port"
+        how = "scalar walk, -O2" + (" (reference templates)" if use_ref else " (oracle)")
+    # sample: the whole level, every sub-grid once per pass
+    sample = np.ascontiguousarray(lv.targets)
+    pairs = pairs_per_subgrid * len(sample)
+    run(sample[:threads], threads)                       # warm-up
+    times = []
+    for _ in range(max(1, args.cpu_repeats)):
+        t = time.perf_counter()
+        run(sample, threads)
+        times.append(time.perf_counter() - t)
+    dt = float(np.median(times))
+    one = np.ascontiguousarray(lv.targets[:32])
+    t = time.perf_counter()
+    run(one, 1)
+    dt1 = time.perf_counter() - t
+    return {
+        "value": pairs / dt, "unit": "cell-interactions/s", "cores": threads, "kind": kind,
+        "sample": (f"all {len(sample)} sub-grids of the level, {pairs} pairs, "
+                   f"median of {len(times)} passes: {dt:.3f} s on {threads} threads = {dt * threads:.1f} core-s "
+                   f"per pass; {how}; one sub-grid per OpenMP task"),
+        "host": hi,
+        "threads_note": f"{threads} threads: the host CPUs a one-GPU process is given on this box "
+                        f"(at most {CPU_THREADS_MAX})",
+        "single_core_value": pairs_per_subgrid * len(one) / dt1,
+        "single_core_sample": f"{len(one)} sub-grids ({pairs_per_subgrid * len(one)} pairs) on 1 thread, {dt1:.3f} s",
     }
 
 
@@ -221,15 +354,22 @@ def main():
         src.free()
         dst.free()
 
-    lv = workloads.config2()
+    # headline: the 4096-sub-grid level
+    lv = workloads.config4()
+    ref = None
+    if not args.no_check:
+        import oracle
+        t = time.perf_counter()
+        ref = oracle.level_interactions(lv, nthreads=min(CPU_THREADS_MAX, len(os.sched_getaffinity(0))))
+        oracle_s = time.perf_counter() - t
     t0 = time.perf_counter()
-    c2, (L64, Lc64) = run_level(ctx, lv, args, hip, _abi)
+    c4 = run_level(ctx, lv, args, hip, _abi, ref=ref)
     wall = time.perf_counter() - t0
-    st = c2["fp64"]["stats"]
+    st = c4["fp64"]["stats"]
     pairs = st["far_pairs"] + st["near_pairs"]
-    ms_step = c2["fp64"]["ms_per_step"]
+    ms_step = c4["fp64"]["ms_per_step"]
     value = pairs / (ms_step * 1e-3)
-    k_ms = c2["fp64"]["stage_ms"][1]
+    k_ms = c4["fp64"]["stage_ms"][1]
     achieved = st["flops"] / (k_ms * 1e-3) / 1e12
 
     traffic = None
@@ -240,68 +380,38 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    check = None
-    if not args.no_check:
-        import oracle
-        sample = lv.targets[:: max(1, len(lv.targets) // 8)][:8]
-        Lr, Lcr = oracle.level_interactions(lv, targets=sample)
-        err = 0.0
-        for a, b in ((L64[sample], Lr[sample]), (Lc64[sample], Lcr[sample])):
-            a2, b2 = a.reshape(-1, a.shape[-1]), b.reshape(-1, b.shape[-1])
-            for c in range(b2.shape[1]):
-                err = max(err, float(np.abs(a2[:, c] - b2[:, c]).max() / np.abs(b2[:, c]).max()))
-        check = {"targets": int(len(sample)), "max_normwise_rel_err_vs_oracle": err}
+    cpu = cpu_baseline(lv, pairs // len(lv.targets), args) if args.cpu_repeats > 0 else None
 
-    cpu = None
-    if args.cpu_repeats > 0:
-        import oracle
-        use_ref = oracle.ref_available()
-        ncores = min(16, os.cpu_count() or 1)
-        t = time.perf_counter()
-        for _ in range(args.cpu_repeats):
-            oracle.level_interactions(lv, nthreads=ncores, use_ref=use_ref)
-        dt = (time.perf_counter() - t) / args.cpu_repeats
-        cpu = {"value": pairs / dt, "unit": "cell-interactions/s", "cores": ncores,
-               "kind": "reference" if use_ref else "port",
-               "sample": (f"the whole level ({len(lv.targets)} sub-grids, {pairs} pairs) x {args.cpu_repeats} passes, "
-                          f"{dt:.3f} s per pass on {ncores} threads = {dt * ncores * args.cpu_repeats:.1f} core-s; "
-                          + ("Octo-Tiger's own FP64 interaction templates (compute_kernel_templates.hpp) compiled "
-                             "with g++ -O2, walked in the Kokkos host-kernel order with SIMD width 1"
-                             if use_ref else "FP64 scalar oracle (restated Kokkos host-kernel walk)")
-                          + ", one sub-grid per OpenMP task")}
-        # one core: the first 32 targets of the level on one thread
-        one = lv.targets[:32]
-        t = time.perf_counter()
-        oracle.level_interactions(lv, targets=one, nthreads=1, use_ref=use_ref)
-        dt1 = time.perf_counter() - t
-        pairs1 = pairs * len(one) / len(lv.targets)
-        cpu["single_core_value"] = pairs1 / dt1
-        cpu["single_core_sample"] = f"{len(one)} sub-grids ({int(pairs1)} pairs) on 1 thread, {dt1:.3f} s"
+    m4 = c4["mixed"]
+    mixed = {"uniform-level-4096": {
+        "value": pairs / (m4["ms_per_step"] * 1e-3), "ms_per_step": m4["ms_per_step"],
+        "speedup_vs_fp64": ms_step / m4["ms_per_step"], "kernel_ms": m4["stage_ms"][1],
+        "kernel_speedup_vs_fp64": k_ms / m4["stage_ms"][1], "kernel_variant": int(m4["stats"]["variant"]),
+        "fp32_far_pairs": int(m4["stats"]["mixed_far_pairs"]),
+        "errors_vs_oracle": m4.get("errors_vs_oracle"), "errors_vs_fp64_pass": m4["errors_vs_fp64_pass"]}}
 
-    mixed = {}
-    m2 = c2["mixed"]
-    mixed["uniform-level-512"] = {
-        "value": pairs / (m2["ms_per_step"] * 1e-3), "ms_per_step": m2["ms_per_step"],
-        "speedup_vs_fp64": ms_step / m2["ms_per_step"], "kernel_ms": m2["stage_ms"][1],
-        "fp32_far_pairs": m2["stats"]["mixed_far_pairs"], "errors": m2["errors"]}
-    if not args.no_c4:
-        lv4 = workloads.config4()
-        c4, _ = run_level(ctx, lv4, args, hip, _abi)
-        s4 = c4["fp64"]["stats"]
-        p4 = s4["far_pairs"] + s4["near_pairs"]
-        mixed["uniform-level-4096"] = {
-            "fp64_value": p4 / (c4["fp64"]["ms_per_step"] * 1e-3), "fp64_ms_per_step": c4["fp64"]["ms_per_step"],
-            "fp64_kernel_ms": c4["fp64"]["stage_ms"][1],
-            "fp64_valu_frac": s4["flops"] / (c4["fp64"]["stage_ms"][1] * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
-            "value": p4 / (c4["mixed"]["ms_per_step"] * 1e-3), "ms_per_step": c4["mixed"]["ms_per_step"],
-            "kernel_ms": c4["mixed"]["stage_ms"][1],
-            "speedup_vs_fp64": c4["fp64"]["ms_per_step"] / c4["mixed"]["ms_per_step"],
-            "fp32_far_pairs": c4["mixed"]["stats"]["mixed_far_pairs"], "errors": c4["mixed"]["errors"]}
+    extra = {}
+    if not args.no_c2:
+        lv2 = workloads.config2()
+        c2 = run_level(ctx, lv2, args, hip, _abi)
+        s2 = c2["fp64"]["stats"]
+        p2 = s2["far_pairs"] + s2["near_pairs"]
+        k2 = c2["fp64"]["stage_ms"][1]
+        extra["uniform-level-512"] = {
+            "value": p2 / (c2["fp64"]["ms_per_step"] * 1e-3), "ms_per_step": c2["fp64"]["ms_per_step"],
+            "kernel_ms": k2, "staging_ms": c2["fp64"]["stage_ms"][0], "kernel_variant": int(s2["variant"]),
+            "kernel": kernel_name(int(s2["variant"])),
+            "valu_frac": s2["flops"] / (k2 * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
+            "mixed": {"ms_per_step": c2["mixed"]["ms_per_step"], "kernel_ms": c2["mixed"]["stage_ms"][1],
+                      "speedup_vs_fp64": c2["fp64"]["ms_per_step"] / c2["mixed"]["ms_per_step"],
+                      "kernel_variant": int(c2["mixed"]["stats"]["variant"]),
+                      "errors_vs_fp64_pass": c2["mixed"]["errors_vs_fp64_pass"]},
+            "production_step": c2["production_step"]}
 
     kinds = None
     if not args.no_c3:
         lv3 = workloads.config3()
-        c3, _ = run_level(ctx, lv3, args, hip, _abi, with_mixed=False)
+        c3 = run_level(ctx, lv3, args, hip, _abi, with_mixed=False)
         s3 = c3["fp64"]["stats"]
         ms3 = c3["fp64"]["ms_per_step"]
         p3 = s3["far_pairs"] + s3["near_pairs"] + s3["p2p_pairs"] + s3["p2m_pairs"]
@@ -310,7 +420,7 @@ def main():
                  "cell_interactions_per_s": p3 / (ms3 * 1e-3),
                  "pairs": {k: int(s3[k]) for k in ("far_pairs", "near_pairs", "p2p_pairs", "p2m_pairs")},
                  "valu_frac": s3["flops"] / (c3["fp64"]["stage_ms"][1] * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
-                 "kernel_variant": int(s3["variant"])}
+                 "kernel_variant": int(s3["variant"]), "production_step": c3["production_step"]}
 
     # dense vector FMA rates of this device: FP64, and FP32 as all device code
     # here is built (packed FP32 off), the rate the mixed-precision pairs get
@@ -336,13 +446,14 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{lv.name}: {len(lv.targets)} non-leaf 8^3 sub-grids with 26-neighbour halos "
-                        f"({lv.nslots} slots), theta=0.5, RHO (angular-momentum correction on), one "
-                        f"hipGraph per pass",
+                        f"({lv.nslots} slots, 144^3 cells), theta=0.5, RHO (angular-momentum correction on), "
+                        f"one hipGraph per pass",
             "subgrids": int(len(lv.targets)),
             "far_pairs_per_pass": int(st["far_pairs"]),
             "flops_per_pair": FLOPS_PER_PAIR["rho"],
             "precision": "fp64",
             "kernel_variant": int(st["variant"]),
+            "plan_create_s": c4["fp64"]["plan_create_s"],
         },
         "roofline": {
             "bound": "valu",
@@ -353,18 +464,23 @@ def main():
             "frac": achieved / FP64_VALU_PEAK_TFLOPS,
             "traffic": traffic,
             "kernel_ms": k_ms,
-            "staging_ms": c2["fp64"]["stage_ms"][0],
+            "staging_ms": c4["fp64"]["stage_ms"][0],
             # the dense FP64 FMA rate of this box, re-measured (octofmm_measure_fma_rate)
             "measured_peak": fma_rate["fp64"],
             "frac_of_measured_peak": achieved / fma_rate["fp64"],
         },
         "cpu_baseline": cpu,
+        "parity_check": ({"targets": int(len(lv.targets)), "oracle_s": oracle_s,
+                          **{k: c4["fp64"]["errors_vs_oracle"][k] for k in
+                             ("max_normwise_rel_err", "force_rel_err", "torque_rel_err")}}
+                         if ref is not None else None),
         "mixed_precision": mixed,
+        "production_step": c4["production_step"],
         "fma_rate_tflops": fma_rate,
+        "uniform_level_512": extra.get("uniform-level-512"),
         "mixed_kind_level": kinds,
         "tree_solve": tree,
-        "parity_check": check,
-        "wall_s_c2": wall,
+        "wall_s_headline": wall,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -15,9 +15,12 @@ from octotiger_amd import _abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboctofmm_oracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libocto_ref.so")
+REF_SIMD_PATHS = {8: os.path.join(HERE, "_ref", "libocto_ref_simd8.so"),
+                  4: os.path.join(HERE, "_ref", "libocto_ref_simd4.so")}
 
 _lib = None
 _ref = None
+_ref_simd = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _u8 = ctypes.POINTER(ctypes.c_uint8)
@@ -67,6 +70,61 @@ def ref_lib():
             ctypes.c_int32, _dp, _dp, _dp, _dp, _dp, ctypes.c_int32]
         _ref = R
     return _ref
+
+
+def host_simd_width():
+    """Vector width (doubles) of the host: 8 with AVX-512F, 4 with AVX2 and
+    FMA, else 0 (read from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = next((ln.split(":", 1)[1].split() for ln in f if ln.startswith("flags")), [])
+    except OSError:
+        return 0
+    if {"avx512f", "avx512dq", "avx512vl", "avx512bw"} <= set(flags):
+        return 8
+    if {"avx2", "fma"} <= set(flags):
+        return 4
+    return 0
+
+
+def ref_simd_available():
+    w = host_simd_width()
+    return w in REF_SIMD_PATHS and os.path.exists(REF_SIMD_PATHS[w])
+
+
+def ref_simd_lib():
+    """The timed CPU baseline build of the reference templates for this host's
+    vector width (oracle/_ref/libocto_ref_simd{8,4}.so)."""
+    global _ref_simd
+    if _ref_simd is None:
+        w = host_simd_width()
+        if w not in REF_SIMD_PATHS or not os.path.exists(REF_SIMD_PATHS[w]):
+            raise RuntimeError(f"no vector build of the reference harness for this host (width {w})")
+        R = ctypes.CDLL(REF_SIMD_PATHS[w])
+        R.ref_simd_width.restype = ctypes.c_int32
+        R.ref_level_interactions_simd.argtypes = [
+            ctypes.c_double, ctypes.c_int32, _ip, _ip, _dp, ctypes.c_double, ctypes.c_int32, _ip,
+            ctypes.c_int32, _dp, _dp, _dp, _dp, _dp, ctypes.c_int32]
+        _ref_simd = R
+    return _ref_simd
+
+
+def level_interactions_simd(level, theta=0.5, type_=_abi.RHO, targets=None, nthreads=None):
+    """The reference's interaction templates walked as the Kokkos host kernel
+    walks them (z-rows of 8 cells as vectors of the host's width), -O3,
+    OpenMP over the target sub-grids: the timed CPU baseline.  Non-leaf
+    targets only.  Returns (L, L_c) shaped like level.M / level.com."""
+    targets = level.targets if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
+    L = np.zeros((level.nslots, 512, 20))
+    L_c = np.zeros((level.nslots, 512, 3))
+    nthreads = nthreads or min(8, os.cpu_count() or 1)
+    st = ref_simd_lib().ref_level_interactions_simd(
+        theta, level.nslots, _abi.iptr(level.neighbors), _abi.iptr(level.kind), _abi.dptr(level.xbase),
+        level.dx, int(targets.size), _abi.iptr(targets), type_, _abi.dptr(level.M),
+        _abi.dptr(level.com), _abi.dptr(level.mon), _abi.dptr(L), _abi.dptr(L_c), nthreads)
+    if st != 0:
+        raise RuntimeError(f"reference SIMD level_interactions failed ({st})")
+    return L, L_c
 
 
 def _u8p(a):

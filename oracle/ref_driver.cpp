@@ -515,3 +515,190 @@ extern "C" int ref_level_interactions(double theta, int32_t nslots, const int32_
     }
     return status;
 }
+
+// ---------------------------------------------------------------------------
+// The timed CPU baseline: the Kokkos host kernels' z-row SIMD walk
+// (multipole_kernel_rho_impl / multipole_kernel_non_rho_impl,
+// octotiger/multipole_interactions/kernel/kokkos_kernel.hpp:137-321, 474-633,
+// launched by launch_interface :865-940 with one z-row of INX cells split into
+// INX / W vectors of W lanes), with the reference's own compute_kernel_rho /
+// compute_kernel_non_rho instantiated on a W-wide vector of doubles in place
+// of the Kokkos SIMD type.  One sub-grid per OpenMP task, staged AoS -> SoA
+// per sub-grid as update_input does; stencils built once per level call, as
+// the reference builds them once at start-up.  The library is built three
+// times by oracle/Makefile: the pinning build (-O2 -ffp-contract=off, no
+// vector walk: REF_SIMD_WIDTH 1) and the timed builds W = 8 (-O3
+// -march=x86-64-v4) and W = 4 (-O3 -march=x86-64-v3), which export
+// ref_level_interactions_simd.
+// ---------------------------------------------------------------------------
+#ifndef REF_SIMD_WIDTH
+#define REF_SIMD_WIDTH 1
+#endif
+
+extern "C" int ref_simd_width(void) { return REF_SIMD_WIDTH; }
+
+#if REF_SIMD_WIDTH > 1
+
+namespace refsimd {
+
+template <int W>
+struct vec {
+    typedef double raw __attribute__((vector_size(W * 8)));
+    typedef long long mask __attribute__((vector_size(W * 8)));
+    raw x;
+    vec() = default;
+    vec(double a) { x = raw{} + a; }
+    static vec from(raw r) {
+        vec v;
+        v.x = r;
+        return v;
+    }
+    vec& operator+=(const vec& o) { x += o.x; return *this; }
+    vec& operator-=(const vec& o) { x -= o.x; return *this; }
+    vec& operator*=(const vec& o) { x *= o.x; return *this; }
+};
+template <int W> vec<W> operator+(const vec<W>& a, const vec<W>& b) { return vec<W>::from(a.x + b.x); }
+template <int W> vec<W> operator-(const vec<W>& a, const vec<W>& b) { return vec<W>::from(a.x - b.x); }
+template <int W> vec<W> operator*(const vec<W>& a, const vec<W>& b) { return vec<W>::from(a.x * b.x); }
+template <int W> vec<W> operator/(const vec<W>& a, const vec<W>& b) { return vec<W>::from(a.x / b.x); }
+template <int W> vec<W> operator+(double a, const vec<W>& b) { return vec<W>::from(a + b.x); }
+template <int W> vec<W> operator-(double a, const vec<W>& b) { return vec<W>::from(a - b.x); }
+template <int W> vec<W> operator*(double a, const vec<W>& b) { return vec<W>::from(a * b.x); }
+template <int W> vec<W> operator/(double a, const vec<W>& b) { return vec<W>::from(a / b.x); }
+template <int W> vec<W> operator+(const vec<W>& a, double b) { return vec<W>::from(a.x + b); }
+template <int W> vec<W> operator-(const vec<W>& a, double b) { return vec<W>::from(a.x - b); }
+template <int W> vec<W> operator*(const vec<W>& a, double b) { return vec<W>::from(a.x * b); }
+template <int W> vec<W> operator/(const vec<W>& a, double b) { return vec<W>::from(a.x / b); }
+template <int W> vec<W> operator-(const vec<W>& a) { return vec<W>::from(-a.x); }
+template <int W> vec<W> sqrt(const vec<W>& a) {
+    vec<W> r;
+    for (int i = 0; i < W; ++i) r.x[i] = __builtin_sqrt(a.x[i]);
+    return r;
+}
+template <int W> vec<W> vmax(const vec<W>& a, const vec<W>& b) { return vec<W>::from(a.x > b.x ? a.x : b.x); }
+template <int W> vec<W> load(const double* p) {
+    vec<W> r;
+    std::memcpy(&r.x, p, sizeof(r.x));
+    return r;
+}
+
+// One z-row segment of W target cells of one sub-grid (kokkos_kernel.hpp:137-321).
+template <int W>
+void row_segment(const RefStencils& s, const RefWindow& w, bool rho, double th2, int idx, int idy, int idz,
+    double* Lout, double* Lcout) {
+    typedef vec<W> V;
+    typedef typename V::mask lraw;
+    const multiindex<> ci(idx + INNER_CELLS_PADDING_DEPTH, idy + INNER_CELLS_PADDING_DEPTH,
+        idz * W + INNER_CELLS_PADDING_DEPTH);
+    const size_t cf = to_flat_index_padded(ci);
+    const int32_t ccx = ((ci.x + INX) >> 1) - (INX / 2);
+    const int32_t ccy = ((ci.y + INX) >> 1) - (INX / 2);
+    int32_t ccz[W];
+    for (int i = 0; i < W; ++i) ccz[i] = ((ci.z + i + INX) >> 1) - (INX / 2);
+    V mc[20], X[3], L[20], C[3], mp[20], Y[3];
+    for (int c = 0; c < 20; ++c) mc[c] = load<W>(&w.M[c * CLEN + cf]);
+    for (int c = 0; c < 3; ++c) X[c] = load<W>(&w.com[c * CLEN + cf]);
+    for (int c = 0; c < 20; ++c) L[c] = V(0.0);
+    for (int c = 0; c < 3; ++c) C[c] = V(0.0);
+    auto vmax_f = [](const V& a, const V& b) -> V { return vmax(a, b); };
+    for (int x = 0; x < STENCIL_INX; ++x) {
+        const int px = ci.x + x + STENCIL_MIN;
+        const int32_t dxc = ccx - (((px + INX) >> 1) - (INX / 2));
+        for (int y = 0; y < STENCIL_INX; ++y) {
+            const int py = ci.y + y + STENCIL_MIN;
+            const int32_t dyc = ccy - (((py + INX) >> 1) - (INX / 2));
+            for (int sz = STENCIL_MIN; sz <= STENCIL_MAX; ++sz) {
+                const size_t index = x * STENCIL_INX * STENCIL_INX + y * STENCIL_INX + (sz - STENCIL_MIN);
+                if (!s.mask[index]) continue;
+                const int pz = ci.z + sz;
+                V tc;
+                for (int i = 0; i < W; ++i) {
+                    const int32_t dzc = ccz[i] - (((pz + i + INX) >> 1) - (INX / 2));
+                    tc.x[i] = static_cast<double>(dxc * dxc + dyc * dyc + dzc * dzc);
+                }
+                const lraw mask = tc.x < th2;
+                bool any = false;
+                for (int i = 0; i < W; ++i) any = any || mask[i] != 0;
+                if (!any) continue;
+                const size_t pf = to_flat_index_padded(multiindex<>(px, py, pz));
+                for (int c = 0; c < 3; ++c) Y[c] = load<W>(&w.com[c * CLEN + pf]);
+                const typename V::raw zero = {};
+                mp[0] = V::from(mask ? load<W>(&w.mon[pf]).x : zero);
+                const lraw fmask = s.ind[index] ? mask : lraw{};
+                mp[0].x += fmask ? load<W>(&w.M[pf]).x : zero;
+                for (int c = 1; c < 20; ++c) mp[c] = V::from(fmask ? load<W>(&w.M[c * CLEN + pf]).x : zero);
+                if (rho)
+                    mi::compute_kernel_rho(X, Y, mp, L, C, mc, vmax_f);
+                else
+                    mi::compute_kernel_non_rho(X, Y, mp, L, vmax_f);
+            }
+        }
+    }
+    for (int i = 0; i < W; ++i) {
+        const size_t u = to_inner_flat_index_not_padded(multiindex<>(idx, idy, idz * W + i));
+        for (int c = 0; c < 20; ++c) Lout[20 * u + c] += L[c].x[i];
+        if (rho)
+            for (int c = 0; c < 3; ++c) Lcout[3 * u + c] = C[c].x[i];
+    }
+}
+
+}   // namespace refsimd
+
+// Non-root M2L over the target slots of a level (non-leaf targets with full
+// or empty neighbours; leaf targets and leaf neighbours go through the scalar
+// walk above), one sub-grid per OpenMP task.
+extern "C" int ref_level_interactions_simd(double theta, int32_t nslots, const int32_t* neighbors,
+    const int32_t* kind, const double* xbase, double dx, int32_t ntargets, const int32_t* targets,
+    int32_t type, const double* M, const double* com, const double* mon, double* L, double* L_c,
+    int32_t nthreads) {
+    static_assert(INX % REF_SIMD_WIDTH == 0, "z-rows split into whole vectors");
+    RefStencils s;
+    if (!ref_stencils(theta, s)) return OCTOFMM_ERR_THETA;
+    const double th2 = (1.0 / theta) * (1.0 / theta);
+    const bool rho = type == OCTOFMM_RHO;
+    int status = OCTOFMM_OK;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int t = 0; t < ntargets; ++t) {
+        const int sl = targets[t];
+        if (sl < 0 || sl >= nslots || kind[sl] != OCTOFMM_SLOT_NONLEAF) {
+#pragma omp critical
+            status = OCTOFMM_ERR_INVALID_ARGUMENT;
+            continue;
+        }
+        octofmm_subgrid in;
+        std::memset(&in, 0, sizeof(in));
+        in.M = M + (size_t)sl * 512 * 20;
+        in.com = com + (size_t)sl * 512 * 3;
+        in.mon = mon + (size_t)sl * 512;
+        in.dx = dx;
+        for (int k = 0; k < 3; ++k) in.xbase[k] = xbase[3 * sl + k];
+        in.type = type;
+        for (int f = 0; f < 26; ++f) {
+            const int raw = f < 13 ? f : f + 1;
+            const int n = neighbors[(size_t)sl * 27 + raw];
+            octofmm_neighbor& nb = in.neighbors[f];
+            if (n < 0) {
+                nb.kind = OCTOFMM_NEIGHBOR_EMPTY;
+            } else if (kind[n] == OCTOFMM_SLOT_LEAF) {
+                nb.kind = OCTOFMM_NEIGHBOR_MONOPOLE;
+                nb.count = 512;
+                nb.mon = mon + (size_t)n * 512;
+            } else {
+                nb.kind = OCTOFMM_NEIGHBOR_MULTIPOLE;
+                nb.count = 512;
+                nb.M = M + (size_t)n * 512 * 20;
+                nb.com = com + (size_t)n * 512 * 3;
+            }
+        }
+        RefWindow w;
+        ref_stage(&in, w);
+        double* Ls = L + (size_t)sl * 512 * 20;
+        double* Lcs = L_c + (size_t)sl * 512 * 3;
+        for (int idx = 0; idx < INX; ++idx)
+            for (int idy = 0; idy < INX; ++idy)
+                for (int idz = 0; idz < INX / REF_SIMD_WIDTH; ++idz)
+                    refsimd::row_segment<REF_SIMD_WIDTH>(s, w, rho, th2, idx, idy, idz, Ls, Lcs);
+    }
+    return status;
+}
+#endif   // REF_SIMD_WIDTH > 1

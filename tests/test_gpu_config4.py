@@ -1,0 +1,77 @@
+"""The 4096-sub-grid uniform level (16^3 non-leaf targets in an 18^3-slot
+block, 144^3 cells, theta = 0.5, RHO) at full size against the CPU FP64
+oracle, in both precisions, each plan autotuned as the bench runs it.
+
+Gates:
+  FP64   per component normwise <= 1e-12; total force and torque of all target
+         cells within 1e-12 of the oracle's totals (|F - F_ref| / |F_ref|).
+  mixed  FP32 pair arithmetic for far offsets with |d|^2 > 4 (cell units),
+         FP64 accumulation and FP64 angular-momentum correction: total force
+         and torque within 1e-6 of the oracle's totals, per component
+         normwise <= 1e-5, L_c normwise <= 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from metrics import force_torque_errors, normwise
+from octotiger_amd import _abi, hip, native, workloads
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+MIXED_FORCE_TOL = 1e-6
+MIXED_COMPONENT_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def c4():
+    lv = workloads.config4()
+    print(f"config4: computing the oracle over {len(lv.targets)} sub-grids", flush=True)
+    Lr, Lcr = oracle.level_interactions(lv)
+    yield lv, Lr, Lcr
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = native.Context(theta=0.5, device=0)
+    yield c
+    c.close()
+
+
+def run(ctx, lv, precision):
+    dl = hip.LevelOnDevice(lv)
+    stream = hip.Stream()
+    plan = dl.plan(ctx, type_=_abi.RHO, precision=precision, mixed_far_d2=4,
+                   flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
+    st = plan.stats()
+    plan.execute(stream.handle)
+    stream.synchronize()
+    L, Lc = dl.L.numpy(), dl.L_c.numpy()
+    plan.close()
+    stream.destroy()
+    dl.free()
+    return L, Lc, st
+
+
+@pytest.mark.parametrize("precision", [_abi.PRECISION_FP64, _abi.PRECISION_MIXED])
+def test_level_config4_full_size(ctx, c4, precision):
+    lv, Lr, Lcr = c4
+    print(f"config4 precision {precision}", flush=True)
+    L, Lc, st = run(ctx, lv, precision)
+    assert st["far_pairs"] == 4096 * 96768 and st["near_pairs"] == 0
+    tg = lv.targets
+    en = normwise(L[tg], Lr[tg])
+    ec = normwise(Lc[tg], Lcr[tg])
+    ft = force_torque_errors(L[tg], Lc[tg], Lr[tg], Lcr[tg], lv.M[tg, :, 0], lv.com[tg])
+    print(f"config4 precision {precision} variant {st['variant']}: normwise {en:.3e} L_c {ec:.3e} "
+          f"force {ft['force']:.3e} torque {ft['torque']:.3e} "
+          f"(per-cell sums: {ft['force_per_cell_sum']:.3e} {ft['torque_per_cell_sum']:.3e})", flush=True)
+    assert ec <= TOL
+    if precision == _abi.PRECISION_FP64:
+        assert en <= TOL
+        assert ft["force"] <= TOL and ft["torque"] <= TOL
+    else:
+        assert st["mixed_far_pairs"] > 0.9 * st["far_pairs"]
+        assert 0 < en <= MIXED_COMPONENT_TOL
+        assert ft["force"] <= MIXED_FORCE_TOL and ft["torque"] <= MIXED_FORCE_TOL

@@ -371,3 +371,20 @@ def test_level_driver_equals_per_subgrid_calls():
         else:
             a, b = oracle.multipole_interactions(sg)
         assert np.array_equal(a, L[s])
+
+
+@pytest.mark.skipif(not oracle.ref_simd_available(), reason="vector build of the reference harness not built")
+@pytest.mark.parametrize("t", [_abi.RHO, _abi.DRHODT])
+def test_vector_reference_baseline_matches_oracle(t):
+    """The timed CPU baseline (the reference templates on host-width vectors,
+    -O3, z-row walk of the Kokkos host kernel) gives the oracle's result to
+    rounding, including targets on the domain edge (empty neighbours)."""
+    from metrics import normwise
+    lv = workloads.uniform_level((3, 3, 3), ((0, 3),) * 3, seed=41)
+    Lr, Lcr = oracle.level_interactions(lv, type_=t)
+    Ls, Lcs = oracle.level_interactions_simd(lv, type_=t)
+    tg = lv.targets
+    assert (lv.neighbors[tg] < 0).any()
+    assert normwise(Ls[tg], Lr[tg]) <= 1e-12
+    if t == _abi.RHO:
+        assert normwise(Lcs[tg], Lcr[tg]) <= 1e-12
