@@ -1387,6 +1387,58 @@ __global__ void zero_fill(double* p, size_t n) {
     if (i < n) p[i] = 0.0;
 }
 
+// Partial halos of the per-sub-grid entry points: a neighbour that hands over
+// only some of its cells (grid::get_ilist_n_bnd order,
+// multipole_interaction_interface.hpp:177,257, p2m_interaction_interface.hpp:142,239)
+// becomes a full slot with zeros in every cell it did not hand over, which is
+// what update_input's zero padding gives.  One block per such neighbour: it
+// zeroes the slot, then writes each listed cell from its position in the
+// handed-over arrays.  The host lists each cell once (its last occurrence),
+// so no two threads write one cell.
+struct ScatterArgs {
+    const double* cM;       // [26][512][20] handed-over multipoles, by partial direction q
+    const double* ccom;     // [26][512][3]
+    const double* cmon;     // [26][512]
+    const int2* cidx;       // [26][512] (cell, position)
+    double* M;              // [27][512][20] slots
+    double* com;            // [27][512][3]
+    double* mon;            // [27][512]
+    int32_t slot[26];       // slot of partial direction q
+    int32_t count[26];      // listed cells of q
+    int32_t multipole[26];  // 1: multipoles and centres of mass, 0: monopoles
+};
+
+__global__ __launch_bounds__(256) void scatter_partial(ScatterArgs a) {
+    const int q = blockIdx.x, t = threadIdx.x;
+    const int s = a.slot[q], n = a.count[q];
+    const bool mp = a.multipole[q] != 0;
+    double* M = a.M + (size_t)s * kCells * 20;
+    double* C = a.com + (size_t)s * kCells * 3;
+    double* m = a.mon + (size_t)s * kCells;
+    if (mp) {
+        for (int i = t; i < kCells * 20; i += 256) M[i] = 0.0;
+        for (int i = t; i < kCells * 3; i += 256) C[i] = 0.0;
+    } else {
+        for (int i = t; i < kCells; i += 256) m[i] = 0.0;
+    }
+    __syncthreads();
+    const int2* idx = a.cidx + (size_t)q * kCells;
+    for (int i = t; i < n; i += 256) {
+        const int2 e = idx[i];
+        if (mp) {
+            const double* src = a.cM + ((size_t)q * kCells + e.y) * 20;
+#pragma unroll
+            for (int c = 0; c < 20; ++c) M[e.x * 20 + c] = src[c];
+            const double* sc = a.ccom + ((size_t)q * kCells + e.y) * 3;
+            C[e.x * 3] = sc[0];
+            C[e.x * 3 + 1] = sc[1];
+            C[e.x * 3 + 2] = sc[2];
+        } else {
+            m[e.x] = a.cmon[(size_t)q * kCells + e.y];
+        }
+    }
+}
+
 // geometric centre xbase + i*dx, rounded as written (grid_fmm.cpp:1669-1671)
 __device__ __forceinline__ double cell_centre(double xb, int i, double dx) {
 #pragma clang fp contract(off)
@@ -1484,6 +1536,76 @@ __global__ __launch_bounds__(256) void m2m_kernel(M2MArgs a) {
 // ===========================================================================
 // Host side
 // ===========================================================================
+namespace {
+
+constexpr int kRootChunks = 64;
+// record buffers end with a tail the direct-to-LDS staging may read into
+// (one 16-byte pad chunk past the last record)
+constexpr size_t kRecTailBytes = 4096;
+// layout of the per-sub-grid workspace metadata (int32 words); the int2 work
+// item comes first so it stays 8-byte aligned
+constexpr int kMetaItems = 0;
+constexpr int kMetaNbr = 2;
+constexpr int kMetaKind = kMetaNbr + 27 * 27;
+constexpr int kMetaFoff = kMetaKind + 27;
+constexpr int kMetaSize = kMetaFoff + 81;
+
+// Device workspace of the per-sub-grid entry points: a 27-slot level (slot =
+// raw direction index, 13 = the target sub-grid), the neighbour data as the
+// caller hands it over, and the root kernel's chunk sums.  Every region starts
+// on a 256-byte boundary of one block.
+struct SubgridWorkspace {
+    double* M = nullptr;        // [27][512][20]
+    double* com = nullptr;      // [27][512][3]
+    double* mon = nullptr;      // [27][512]
+    double* rec = nullptr;      // [27][512][24] + tail
+    double* L = nullptr;        // [27][512][20]: slot 13 holds the target's L
+    double* Lc = nullptr;       // [27][512][3]
+    double* partial = nullptr;  // [kRootChunks][512][24]
+    double* cM = nullptr;       // [26][512][20] partial halos as handed over, by partial direction
+    double* ccom = nullptr;     // [26][512][3]
+    double* cmon = nullptr;     // [26][512]
+    int2* cidx = nullptr;       // [26][512] (cell, position in the handed-over arrays)
+    int32_t* meta = nullptr;    // kMetaSize
+    double* fx = nullptr;       // [27][3]
+};
+
+constexpr size_t ws_align(size_t n) { return (n + 255) & ~size_t(255); }
+constexpr size_t kWsM = 0;
+constexpr size_t kWsCom = kWsM + ws_align(sizeof(double) * 27 * kCells * 20);
+constexpr size_t kWsMon = kWsCom + ws_align(sizeof(double) * 27 * kCells * 3);
+constexpr size_t kWsRec = kWsMon + ws_align(sizeof(double) * 27 * kCells);
+constexpr size_t kWsL = kWsRec + ws_align(sizeof(double) * 27 * kCells * kRecDoubles + kRecTailBytes);
+constexpr size_t kWsLc = kWsL + ws_align(sizeof(double) * 27 * kCells * 20);
+constexpr size_t kWsPartial = kWsLc + ws_align(sizeof(double) * 27 * kCells * 3);
+constexpr size_t kWsCM = kWsPartial + ws_align(sizeof(double) * kRootChunks * kCells * kRecDoubles);
+constexpr size_t kWsCCom = kWsCM + ws_align(sizeof(double) * 26 * kCells * 20);
+constexpr size_t kWsCMon = kWsCCom + ws_align(sizeof(double) * 26 * kCells * 3);
+constexpr size_t kWsCIdx = kWsCMon + ws_align(sizeof(double) * 26 * kCells);
+constexpr size_t kWsMeta = kWsCIdx + ws_align(sizeof(int2) * 26 * kCells);
+constexpr size_t kWsFx = kWsMeta + ws_align(sizeof(int32_t) * kMetaSize);
+constexpr size_t kWsBytes = kWsFx + ws_align(sizeof(double) * 81);
+
+SubgridWorkspace carve_workspace(char* base) {
+    SubgridWorkspace w;
+    w.M = reinterpret_cast<double*>(base + kWsM);
+    w.com = reinterpret_cast<double*>(base + kWsCom);
+    w.mon = reinterpret_cast<double*>(base + kWsMon);
+    w.rec = reinterpret_cast<double*>(base + kWsRec);
+    w.L = reinterpret_cast<double*>(base + kWsL);
+    w.Lc = reinterpret_cast<double*>(base + kWsLc);
+    w.partial = reinterpret_cast<double*>(base + kWsPartial);
+    w.cM = reinterpret_cast<double*>(base + kWsCM);
+    w.ccom = reinterpret_cast<double*>(base + kWsCCom);
+    w.cmon = reinterpret_cast<double*>(base + kWsCMon);
+    w.cidx = reinterpret_cast<int2*>(base + kWsCIdx);
+    w.meta = reinterpret_cast<int32_t*>(base + kWsMeta);
+    w.fx = reinterpret_cast<double*>(base + kWsFx);
+    return w;
+}
+
+}   // namespace
+
 struct octofmm_context {
     int device = 0;
     double theta = 0.5;
@@ -1492,23 +1614,13 @@ struct octofmm_context {
     Tables* d_tab = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
-    // per-sub-grid workspace: a 27-slot level (slot = raw direction index)
-    bool ws_ready = false;
-    double* h_M = nullptr;     // pinned [27][512][20]
-    double* h_com = nullptr;   // pinned [27][512][3]
-    double* h_mon = nullptr;   // pinned [27][512]
-    double* h_out = nullptr;   // pinned [512][23]
-    int32_t* h_meta = nullptr; // pinned: items[2] nbr[27*27] kind[27] foff[27*3]
-    double* h_fx = nullptr;    // pinned [27][3]
-    double* d_M = nullptr;
-    double* d_com = nullptr;
-    double* d_mon = nullptr;
-    double* d_rec = nullptr;
-    double* d_L = nullptr;     // [27][512][20]
-    double* d_Lc = nullptr;    // [27][512][3]
-    int32_t* d_meta = nullptr;
-    double* d_fx = nullptr;
-    double* d_partial = nullptr;   // root chunks
+    // per-sub-grid entry points: the device workspace (one allocation on the
+    // first call) and the host-side metadata they upload, built at creation
+    SubgridWorkspace ws;
+    char* d_ws = nullptr;
+    int32_t* h_meta = nullptr;     // items[2] nbr[27*27] kind[27] foff[27*3]
+    double* h_fx = nullptr;        // [27][3]
+    int2* h_idx = nullptr;         // [26][512] (cell, position) of partial halos
     // p2m pair lists by direction (built from the tables at creation)
     int32_t* d_p2m_dir = nullptr;
     int4* d_p2m_tgt = nullptr;
@@ -1563,18 +1675,6 @@ struct octofmm_plan {
 
 namespace {
 
-constexpr int kRootChunks = 64;
-// record buffers end with a tail the direct-to-LDS staging may read into
-// (one 16-byte pad chunk past the last record)
-constexpr size_t kRecTailBytes = 4096;
-// layout of the per-sub-grid workspace metadata (int32 words); the int2 work
-// item comes first so it stays 8-byte aligned
-constexpr int kMetaItems = 0;
-constexpr int kMetaNbr = 2;
-constexpr int kMetaKind = kMetaNbr + 27 * 27;
-constexpr int kMetaFoff = kMetaKind + 27;
-constexpr int kMetaSize = kMetaFoff + 81;
-
 octofmm_status fail(octofmm_context* c, octofmm_status s, const char* fmt, ...) {
     if (c) {
         char buf[512];
@@ -1597,24 +1697,11 @@ octofmm_status fail(octofmm_context* c, octofmm_status s, const char* fmt, ...) 
 
 int raw_of_flat(int f) { return f < 13 ? f : f + 1; }
 
+// The per-sub-grid workspace of a context, allocated on its first call.
 octofmm_status ensure_workspace(octofmm_context* c) {
-    if (c->ws_ready) return OCTOFMM_OK;
-    HIPC(c, hipHostMalloc((void**)&c->h_M, sizeof(double) * 27 * kCells * 20, hipHostMallocDefault));
-    HIPC(c, hipHostMalloc((void**)&c->h_com, sizeof(double) * 27 * kCells * 3, hipHostMallocDefault));
-    HIPC(c, hipHostMalloc((void**)&c->h_mon, sizeof(double) * 27 * kCells, hipHostMallocDefault));
-    HIPC(c, hipHostMalloc((void**)&c->h_out, sizeof(double) * kCells * 23, hipHostMallocDefault));
-    HIPC(c, hipHostMalloc((void**)&c->h_meta, sizeof(int32_t) * kMetaSize, hipHostMallocDefault));
-    HIPC(c, hipHostMalloc((void**)&c->h_fx, sizeof(double) * 81, hipHostMallocDefault));
-    HIPC(c, hipMalloc((void**)&c->d_M, sizeof(double) * 27 * kCells * 20));
-    HIPC(c, hipMalloc((void**)&c->d_com, sizeof(double) * 27 * kCells * 3));
-    HIPC(c, hipMalloc((void**)&c->d_mon, sizeof(double) * 27 * kCells));
-    HIPC(c, hipMalloc((void**)&c->d_rec, sizeof(double) * 27 * kCells * kRecDoubles + kRecTailBytes));
-    HIPC(c, hipMalloc((void**)&c->d_L, sizeof(double) * 27 * kCells * 20));
-    HIPC(c, hipMalloc((void**)&c->d_Lc, sizeof(double) * 27 * kCells * 3));
-    HIPC(c, hipMalloc((void**)&c->d_meta, sizeof(int32_t) * kMetaSize));
-    HIPC(c, hipMalloc((void**)&c->d_fx, sizeof(double) * 81));
-    HIPC(c, hipMalloc((void**)&c->d_partial, sizeof(double) * kRootChunks * kCells * kRecDoubles));
-    c->ws_ready = true;
+    if (c->d_ws) return OCTOFMM_OK;
+    HIPC(c, hipMalloc((void**)&c->d_ws, kWsBytes));
+    c->ws = carve_workspace(c->d_ws);
     return OCTOFMM_OK;
 }
 
@@ -1657,73 +1744,103 @@ octofmm_status check_finite_inputs(octofmm_context* c, const octofmm_subgrid* in
     return OCTOFMM_OK;
 }
 
-// Fills the 27-slot staging buffers from a sub-grid descriptor.  Partial
-// halos are expanded to full slots (zeros where no data was handed over),
-// which is what update_input does with its zero padding.
-octofmm_status fill_workspace(octofmm_context* c, const octofmm_subgrid* in, bool leaf_target,
-    bool root) {
-    int32_t* nbr = c->h_meta + kMetaNbr;  // only row 13 is used
-    int32_t* kind = c->h_meta + kMetaKind;
-    int32_t* foff = kind + 27;
+// Loads one sub-grid descriptor into a workspace, on the context's stream:
+// the target's own arrays and every full neighbour are copied from the
+// caller's memory straight into their slots; a partial halo is copied as
+// handed over into its compact region and scatter_partial writes it into its
+// zeroed slot; then the 27-slot metadata.  Slots the metadata does not list
+// are never read, and staging reads only M and com of non-leaf slots and mon
+// of leaf slots, so nothing else needs clearing between calls.  The host
+// buffers read by the copies (c->h_meta, h_fx, h_idx) stay untouched until
+// the caller synchronises the stream.
+octofmm_status load_subgrid(octofmm_context* c, const SubgridWorkspace& w, const octofmm_subgrid* in,
+    bool leaf_target, bool root, int item_flags) {
+    hipStream_t st = c->stream;
+    int32_t* meta = c->h_meta;
+    int32_t* items = meta + kMetaItems;
+    int32_t* nbr = meta + kMetaNbr;  // only row 13 is used
+    int32_t* kind = meta + kMetaKind;
+    int32_t* foff = meta + kMetaFoff;
+    items[0] = 13;
+    items[1] = item_flags;
     for (int i = 0; i < 27 * 27; ++i) nbr[i] = -1;
-    for (int s = 0; s < 27; ++s) {
-        kind[s] = -1;
-        const int d[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
+    for (int sl = 0; sl < 27; ++sl) {
+        kind[sl] = -1;
+        const int d[3] = {sl % 3 - 1, (sl / 3) % 3 - 1, sl / 9 - 1};
         for (int k = 0; k < 3; ++k) {
-            foff[3 * s + k] = kInx + kInx * d[k];
-            c->h_fx[3 * s + k] = in->xbase[k];
+            foff[3 * sl + k] = kInx + kInx * d[k];
+            c->h_fx[3 * sl + k] = in->xbase[k];
         }
     }
-    double* M13 = c->h_M + 13 * kCells * 20;
-    double* C13 = c->h_com + 13 * kCells * 3;
-    double* m13 = c->h_mon + 13 * kCells;
+    const size_t nM = (size_t)kCells * 20, nC = (size_t)kCells * 3;
     if (leaf_target) {
-        std::memset(M13, 0, sizeof(double) * kCells * 20);
-        std::memcpy(m13, in->mon, sizeof(double) * kCells);
-        std::memcpy(C13, in->com, sizeof(double) * kCells * 3);
+        HIPC(c, hipMemcpyAsync(w.mon + 13 * kCells, in->mon, sizeof(double) * kCells, hipMemcpyHostToDevice, st));
+        HIPC(c, hipMemcpyAsync(w.com + 13 * nC, in->com, sizeof(double) * nC, hipMemcpyHostToDevice, st));
         kind[13] = OCTOFMM_SLOT_LEAF;
     } else {
-        std::memcpy(M13, in->M, sizeof(double) * kCells * 20);
-        std::memcpy(C13, in->com, sizeof(double) * kCells * 3);
-        std::memset(m13, 0, sizeof(double) * kCells);
+        HIPC(c, hipMemcpyAsync(w.M + 13 * nM, in->M, sizeof(double) * nM, hipMemcpyHostToDevice, st));
+        HIPC(c, hipMemcpyAsync(w.com + 13 * nC, in->com, sizeof(double) * nC, hipMemcpyHostToDevice, st));
         kind[13] = OCTOFMM_SLOT_NONLEAF;
     }
     nbr[13 * 27 + 13] = 13;
-    if (root) return OCTOFMM_OK;
-    for (int f = 0; f < 26; ++f) {
+    ScatterArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    int np = 0;
+    for (int f = 0; f < (root ? 0 : 26); ++f) {
         const octofmm_neighbor& nb = in->neighbors[f];
-        const int s = raw_of_flat(f);
-        double* Ms = c->h_M + (size_t)s * kCells * 20;
-        double* Cs = c->h_com + (size_t)s * kCells * 3;
-        double* ms = c->h_mon + (size_t)s * kCells;
-        if (nb.kind == OCTOFMM_NEIGHBOR_MULTIPOLE && nb.M && nb.com) {
-            if (nb.count == kCells && !nb.cells) {
-                std::memcpy(Ms, nb.M, sizeof(double) * kCells * 20);
-                std::memcpy(Cs, nb.com, sizeof(double) * kCells * 3);
+        const int sl = raw_of_flat(f);
+        const bool multipole = nb.kind == OCTOFMM_NEIGHBOR_MULTIPOLE && nb.M && nb.com;
+        const bool monopole = nb.kind == OCTOFMM_NEIGHBOR_MONOPOLE && nb.mon;
+        if (!multipole && !monopole) continue;   // empty, or refined without data: the reference skips it
+        if (nb.count == kCells && !nb.cells) {
+            if (multipole) {
+                HIPC(c, hipMemcpyAsync(w.M + sl * nM, nb.M, sizeof(double) * nM, hipMemcpyHostToDevice, st));
+                HIPC(c, hipMemcpyAsync(w.com + sl * nC, nb.com, sizeof(double) * nC, hipMemcpyHostToDevice, st));
             } else {
-                std::memset(Ms, 0, sizeof(double) * kCells * 20);
-                std::memset(Cs, 0, sizeof(double) * kCells * 3);
-                for (int i = 0; i < nb.count; ++i) {
-                    std::memcpy(Ms + 20 * nb.cells[i], nb.M + 20 * i, sizeof(double) * 20);
-                    std::memcpy(Cs + 3 * nb.cells[i], nb.com + 3 * i, sizeof(double) * 3);
+                HIPC(c, hipMemcpyAsync(w.mon + sl * kCells, nb.mon, sizeof(double) * kCells, hipMemcpyHostToDevice, st));
+            }
+        } else {
+            // each listed cell once, from its last occurrence (later entries win,
+            // as in update_input's loop over the list)
+            int16_t last[kCells];
+            for (int i = 0; i < kCells; ++i) last[i] = -1;
+            for (int i = 0; i < nb.count; ++i) last[nb.cells[i]] = (int16_t)i;
+            int2* idx = c->h_idx + (size_t)np * kCells;
+            int n = 0;
+            for (int cell = 0; cell < kCells; ++cell)
+                if (last[cell] >= 0) idx[n++] = make_int2(cell, last[cell]);
+            if (nb.count > 0) {
+                if (multipole) {
+                    HIPC(c, hipMemcpyAsync(w.cM + np * nM, nb.M, sizeof(double) * 20 * nb.count,
+                        hipMemcpyHostToDevice, st));
+                    HIPC(c, hipMemcpyAsync(w.ccom + np * nC, nb.com, sizeof(double) * 3 * nb.count,
+                        hipMemcpyHostToDevice, st));
+                } else {
+                    HIPC(c, hipMemcpyAsync(w.cmon + (size_t)np * kCells, nb.mon, sizeof(double) * nb.count,
+                        hipMemcpyHostToDevice, st));
                 }
             }
-            std::memset(ms, 0, sizeof(double) * kCells);
-            kind[s] = OCTOFMM_SLOT_NONLEAF;
-            nbr[13 * 27 + s] = s;
-        } else if (nb.kind == OCTOFMM_NEIGHBOR_MONOPOLE && nb.mon) {
-            if (nb.count == kCells && !nb.cells) {
-                std::memcpy(ms, nb.mon, sizeof(double) * kCells);
-            } else {
-                std::memset(ms, 0, sizeof(double) * kCells);
-                for (int i = 0; i < nb.count; ++i) ms[nb.cells[i]] = nb.mon[i];
-            }
-            kind[s] = OCTOFMM_SLOT_LEAF;
-            nbr[13 * 27 + s] = s;
-        } else if (leaf_target && nb.kind == OCTOFMM_NEIGHBOR_MULTIPOLE) {
-            // refined neighbour without data: the reference skips its p2m
-            kind[s] = -1;
+            sa.slot[np] = sl;
+            sa.count[np] = n;
+            sa.multipole[np] = multipole ? 1 : 0;
+            ++np;
         }
+        kind[sl] = multipole ? OCTOFMM_SLOT_NONLEAF : OCTOFMM_SLOT_LEAF;
+        nbr[13 * 27 + sl] = sl;
+    }
+    HIPC(c, hipMemcpyAsync(w.meta, meta, sizeof(int32_t) * kMetaSize, hipMemcpyHostToDevice, st));
+    HIPC(c, hipMemcpyAsync(w.fx, c->h_fx, sizeof(double) * 81, hipMemcpyHostToDevice, st));
+    if (np > 0) {
+        HIPC(c, hipMemcpyAsync(w.cidx, c->h_idx, sizeof(int2) * kCells * np, hipMemcpyHostToDevice, st));
+        sa.cM = w.cM;
+        sa.ccom = w.ccom;
+        sa.cmon = w.cmon;
+        sa.cidx = w.cidx;
+        sa.M = w.M;
+        sa.com = w.com;
+        sa.mon = w.mon;
+        hipLaunchKernelGGL(scatter_partial, dim3(np), dim3(256), 0, st, sa);
+        HIPC(c, hipGetLastError());
     }
     return OCTOFMM_OK;
 }
@@ -1795,61 +1912,38 @@ void build_p2m_lists(const Tables& T, std::vector<int32_t>& dir, std::vector<int
             if (!lists[r][t].empty() && !lists[26 - r][t].empty()) opp_disjoint = false;
 }
 
-// Uploads the staged 27-slot level and its tables.
-octofmm_status upload_workspace(octofmm_context* c, int item_flags) {
-    int32_t* meta = c->h_meta;
-    int32_t* items = meta + kMetaItems;
-    items[0] = 13;
-    items[1] = item_flags;
-    hipStream_t st = c->stream;
-    // slots actually used (13 + present neighbours) are copied whole
-    const int32_t* kind = meta + kMetaKind;
-    for (int s = 0; s < 27; ++s) {
-        if (kind[s] < 0) continue;
-        HIPC(c, hipMemcpyAsync(c->d_M + (size_t)s * kCells * 20, c->h_M + (size_t)s * kCells * 20,
-            sizeof(double) * kCells * 20, hipMemcpyHostToDevice, st));
-        HIPC(c, hipMemcpyAsync(c->d_com + (size_t)s * kCells * 3, c->h_com + (size_t)s * kCells * 3,
-            sizeof(double) * kCells * 3, hipMemcpyHostToDevice, st));
-        HIPC(c, hipMemcpyAsync(c->d_mon + (size_t)s * kCells, c->h_mon + (size_t)s * kCells,
-            sizeof(double) * kCells, hipMemcpyHostToDevice, st));
-    }
-    HIPC(c, hipMemcpyAsync(c->d_meta, meta, sizeof(int32_t) * kMetaSize, hipMemcpyHostToDevice, st));
-    HIPC(c, hipMemcpyAsync(c->d_fx, c->h_fx, sizeof(double) * 81, hipMemcpyHostToDevice, st));
-    return OCTOFMM_OK;
-}
-
 }   // namespace
 
-// Runs one sub-grid through the 27-slot workspace and leaves L (20) and
-// L_c (3) per cell in h_out.
-static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, int item_flags, bool root) {
-    octofmm_status s = upload_workspace(c, item_flags);
-    if (s != OCTOFMM_OK) return s;
+// Runs one loaded sub-grid through a workspace: the caller's L is copied in
+// and the kernels add the sub-grid's interactions to it (the same single
+// addition per component a host-side L += result makes), then L comes back,
+// and L_c (when the call writes it) is copied out; synchronous.
+static octofmm_status launch_subgrid(octofmm_context* c, const SubgridWorkspace& w, double dx, int type,
+    int item_flags, bool root, double* L, double* L_c) {
     hipStream_t st = c->stream;
     const int rho = type == OCTOFMM_RHO;
     StageArgs sa;
-    sa.M = c->d_M;
-    sa.com = c->d_com;
-    sa.mon = c->d_mon;
-    sa.kind = c->d_meta + kMetaKind;
-    sa.fxbase = c->d_fx;
-    sa.foff = c->d_meta + kMetaFoff;
-    sa.rec = c->d_rec;
+    sa.M = w.M;
+    sa.com = w.com;
+    sa.mon = w.mon;
+    sa.kind = w.meta + kMetaKind;
+    sa.fxbase = w.fx;
+    sa.foff = w.meta + kMetaFoff;
+    sa.rec = w.rec;
     sa.list = nullptr;
     sa.dx = dx;
     sa.nslots = 27;
     hipLaunchKernelGGL(stage_records, dim3(27 * 4), dim3(256), 0, st, sa);
     HIPC(c, hipGetLastError());
     const size_t nL = (size_t)kCells * 20, nC = (size_t)kCells * 3;
-    double* L13 = c->d_L + 13 * nL;
-    double* C13 = c->d_Lc + 13 * nC;
-    hipLaunchKernelGGL(zero_fill, dim3((nL + 255) / 256), dim3(256), 0, st, L13, nL);
-    hipLaunchKernelGGL(zero_fill, dim3((nC + 255) / 256), dim3(256), 0, st, C13, nC);
+    double* L13 = w.L + 13 * nL;
+    double* C13 = w.Lc + 13 * nC;
+    HIPC(c, hipMemcpyAsync(L13, L, sizeof(double) * nL, hipMemcpyHostToDevice, st));
     if (root) {
         RootArgs ra;
-        ra.rec = c->d_rec + (size_t)13 * kCells * kRecDoubles;
+        ra.rec = w.rec + (size_t)13 * kCells * kRecDoubles;
         ra.tab = c->d_tab;
-        ra.partial = c->d_partial;
+        ra.partial = w.partial;
         ra.per_chunk = kCells / kRootChunks;
         ra.nchunks = kRootChunks;
         if (rho)
@@ -1858,7 +1952,7 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
             hipLaunchKernelGGL(root_kernel<false>, dim3(kRootChunks), dim3(512), 0, st, ra);
         HIPC(c, hipGetLastError());
         RootReduceArgs rr;
-        rr.partial = c->d_partial;
+        rr.partial = w.partial;
         rr.L = L13;
         rr.Lc = C13;
         rr.nchunks = kRootChunks;
@@ -1868,16 +1962,16 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         HIPC(c, hipGetLastError());
     } else {
         LevelArgs la;
-        la.rec = c->d_rec;
-        la.com = c->d_com;
-        la.L = c->d_L;
-        la.Lc = c->d_Lc;
-        la.nbr = c->d_meta + kMetaNbr;
-        la.kind = c->d_meta + kMetaKind;
-        la.items = reinterpret_cast<const int2*>(c->d_meta + kMetaItems);
+        la.rec = w.rec;
+        la.com = w.com;
+        la.L = w.L;
+        la.Lc = w.Lc;
+        la.nbr = w.meta + kMetaNbr;
+        la.kind = w.meta + kMetaKind;
+        la.items = reinterpret_cast<const int2*>(w.meta + kMetaItems);
         la.tab = c->d_tab;
         la.mixbits = nullptr;
-        la.mon = c->d_mon;
+        la.mon = w.mon;
         la.p2m_dir = c->d_p2m_dir;
         la.p2m_tgt = c->d_p2m_tgt;
         la.p2m_part = c->d_p2m_part;
@@ -1907,9 +2001,8 @@ static octofmm_status launch_subgrid(octofmm_context* c, double dx, int type, in
         }
         HIPC(c, hipGetLastError());
     }
-    // results: L (20) then L_c (3) per cell into h_out
-    HIPC(c, hipMemcpyAsync(c->h_out, L13, sizeof(double) * nL, hipMemcpyDeviceToHost, st));
-    HIPC(c, hipMemcpyAsync(c->h_out + nL, C13, sizeof(double) * nC, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipMemcpyAsync(L, L13, sizeof(double) * nL, hipMemcpyDeviceToHost, st));
+    if (L_c) HIPC(c, hipMemcpyAsync(L_c, C13, sizeof(double) * nC, hipMemcpyDeviceToHost, st));
     HIPC(c, hipStreamSynchronize(st));
     return OCTOFMM_OK;
 }
@@ -1947,6 +2040,10 @@ octofmm_status octofmm_create(const octofmm_config* cfg, octofmm_context** out) 
     c->device = cfg->device;
     c->theta = cfg->theta;
     c->flags = cfg->flags;
+    c->err.reserve(600);   // error texts (at most 511 characters) are stored without allocating
+    c->h_meta = new int32_t[kMetaSize];
+    c->h_fx = new double[81];
+    c->h_idx = new int2[26 * kCells];
     char err[256];
     if (!build_tables(cfg->theta, c->tab, err, sizeof(err))) {
         delete c;
@@ -1988,21 +2085,10 @@ octofmm_status octofmm_destroy(octofmm_context* c) {
     if (!c) return OCTOFMM_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    (void)hipHostFree(c->h_M);
-    (void)hipHostFree(c->h_com);
-    (void)hipHostFree(c->h_mon);
-    (void)hipHostFree(c->h_out);
-    (void)hipHostFree(c->h_meta);
-    (void)hipHostFree(c->h_fx);
-    (void)hipFree(c->d_M);
-    (void)hipFree(c->d_com);
-    (void)hipFree(c->d_mon);
-    (void)hipFree(c->d_rec);
-    (void)hipFree(c->d_L);
-    (void)hipFree(c->d_Lc);
-    (void)hipFree(c->d_meta);
-    (void)hipFree(c->d_fx);
-    (void)hipFree(c->d_partial);
+    delete[] c->h_meta;
+    delete[] c->h_fx;
+    delete[] c->h_idx;
+    (void)hipFree(c->d_ws);
     (void)hipFree(c->d_tab);
     (void)hipFree(c->d_p2m_dir);
     (void)hipFree(c->d_p2m_tgt);
@@ -2037,15 +2123,9 @@ octofmm_status octofmm_compute_multipole_interactions(octofmm_context* c, const 
     (void)hipGetLastError();   // launch checks below report this call's launches only
     s = ensure_workspace(c);
     if (s != OCTOFMM_OK) return s;
-    s = fill_workspace(c, in, false, root);
+    s = load_subgrid(c, c->ws, in, false, root, flags);
     if (s != OCTOFMM_OK) return s;
-    s = launch_subgrid(c, in->dx, in->type, flags, root);
-    if (s != OCTOFMM_OK) return s;
-    const double* r = c->h_out;
-    for (int i = 0; i < kCells * 20; ++i) L[i] += r[i];
-    if (in->type == OCTOFMM_RHO)
-        for (int i = 0; i < kCells * 3; ++i) L_c[i] = r[kCells * 20 + i];
-    return OCTOFMM_OK;
+    return launch_subgrid(c, c->ws, in->dx, in->type, flags, root, L, in->type == OCTOFMM_RHO ? L_c : nullptr);
 }
 
 octofmm_status octofmm_compute_monopole_interactions(octofmm_context* c, const octofmm_subgrid* in,
@@ -2067,16 +2147,11 @@ octofmm_status octofmm_compute_monopole_interactions(octofmm_context* c, const o
     (void)hipGetLastError();   // launch checks below report this call's launches only
     s = ensure_workspace(c);
     if (s != OCTOFMM_OK) return s;
-    s = fill_workspace(c, in, true, false);
+    const int flags = kItemLeaf | (p2m ? kItemP2M : 0);
+    s = load_subgrid(c, c->ws, in, true, false, flags);
     if (s != OCTOFMM_OK) return s;
-    s = launch_subgrid(c, in->dx, in->type, kItemLeaf | (p2m ? kItemP2M : 0), false);
-    if (s != OCTOFMM_OK) return s;
-    const double* r = c->h_out;
-    for (int u = 0; u < kCells; ++u)
-        for (int i = 0; i < 4; ++i) L[20 * u + i] += r[20 * u + i];
-    if (in->type == OCTOFMM_RHO && p2m)
-        for (int i = 0; i < kCells * 3; ++i) L_c[i] = r[kCells * 20 + i];
-    return OCTOFMM_OK;
+    return launch_subgrid(c, c->ws, in->dx, in->type, flags, false, L,
+        in->type == OCTOFMM_RHO && p2m ? L_c : nullptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -4,9 +4,13 @@
 //
 // input : int32 target(0 multipole, 1 monopole) type root contains_multipole_neighbor,
 //         double dx xbase[3], then per raw direction 0..26:
-//         int32 kind (-1 empty, 0 non-leaf, 1 leaf), double M[512*20] com[512*3] mon[512]
+//         int32 kind (-1 empty, 0 non-leaf, 1 leaf), int32 n, int32 cells[n],
+//         double M[n*20] com[n*3] mon[n]
+//         (n = 512 with cells 0..511: a full sub-grid; n < 512: a partial halo whose
+//         cells the stand-in grid lists in get_ilist_n_bnd, in the order of the arrays)
 // output: double L[512*20] L_c[512*3]
 #include <array>
+#include <cstdint>
 #include <cstdio>
 #include <fstream>
 #include <memory>
@@ -35,12 +39,18 @@ struct neighbor_gravity_type {
     int direction = 0;
 };
 enum gsolve_type { RHO, DRHODT };
+// grid::get_ilist_n_bnd's element type: `second` is the neighbour-side cell
+// index (octotiger/interaction_types.hpp:134-150)
+struct boundary_interaction_type {
+    std::uint16_t second = 0;
+};
 struct grid {
     std::vector<expansion> L = std::vector<expansion>(512);
     std::vector<space_vector> L_c = std::vector<space_vector>(512);
+    std::vector<boundary_interaction_type> ilist[27];
     std::vector<expansion>& get_L() { return L; }
     std::vector<space_vector>& get_L_c() { return L_c; }
-    std::vector<std::pair<int, int>> get_ilist_n_bnd(int) { return {}; }
+    const std::vector<boundary_interaction_type>& get_ilist_n_bnd(int dir) { return ilist[dir]; }
 };
 
 int main(int argc, char** argv) {
@@ -55,14 +65,25 @@ int main(int argc, char** argv) {
     std::vector<space_vector> X[27];
     std::vector<double> mon[27];
     int32_t kind[27];
+    auto g = std::make_shared<grid>();
     for (int s = 0; s < 27; ++s) {
+        int32_t n = 0;
         in.read((char*)&kind[s], 4);
-        M[s].resize(512);
-        X[s].resize(512);
-        mon[s].resize(512);
-        in.read((char*)M[s].data(), 512 * 20 * 8);
-        in.read((char*)X[s].data(), 512 * 3 * 8);
-        in.read((char*)mon[s].data(), 512 * 8);
+        in.read((char*)&n, 4);
+        if (n < 0 || n > 512) return 3;
+        std::vector<int32_t> cells(n);
+        in.read((char*)cells.data(), 4 * n);
+        M[s].resize(n);
+        X[s].resize(n);
+        mon[s].resize(n);
+        in.read((char*)M[s].data(), n * 20 * 8);
+        in.read((char*)X[s].data(), n * 3 * 8);
+        in.read((char*)mon[s].data(), n * 8);
+        for (int32_t c : cells) {
+            boundary_interaction_type b;
+            b.second = static_cast<std::uint16_t>(c);
+            g->ilist[s].push_back(b);
+        }
     }
     if (!in) return 3;
     std::vector<neighbor_gravity_type> nbrs(26);
@@ -83,7 +104,6 @@ int main(int argc, char** argv) {
     const gsolve_type type = hdr[1] == 0 ? RHO : DRHODT;
     try {
         octofmm_adapter::context ctx(0.5, 0);
-        auto g = std::make_shared<grid>();
         if (hdr[0] == 0) {
             octofmm_adapter::multipole_interaction_interface<grid> mi(ctx);
             mi.set_grid_ptr(g);

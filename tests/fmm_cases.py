@@ -101,3 +101,81 @@ def input_digest(name):
     for a in (lv.M, lv.com, lv.mon, lv.neighbors, lv.kind, lv.xbase):
         h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()
+
+
+def partial_subgrid(level, spec, type_=_abi.RHO, slot=13, stale=None):
+    """Sub-grid descriptor of a slot in which the neighbours named in spec
+    (raw direction -> int32 cell list, in the order the handed-over arrays
+    hold them; repeats allowed) hand over only those cells, as a remote
+    neighbour does with grid::get_ilist_n_bnd.  stale: raw direction -> the
+    number of leading entries whose data are scaled by 3, so that only a
+    staging in which later entries overwrite earlier ones gives the level's
+    values
+    (octotiger/multipole_interactions/legacy/multipole_interaction_interface.hpp:177,257,
+    octotiger/monopole_interactions/legacy/p2m_interaction_interface.hpp:142,239)."""
+    sg, keep = workloads.subgrid_view(level, slot, type_=type_)
+    for raw, cells in spec.items():
+        n = int(level.neighbors[slot, raw])
+        assert n >= 0, "partial halo of an empty direction"
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        nb = sg.neighbors[_abi.raw_to_flat(raw)]
+        nb.count, nb.cells = int(cells.size), _abi.iptr(cells)
+        keep.append(cells)
+        k = (stale or {}).get(raw, 0)
+        if level.kind[n] == _abi.SLOT_LEAF:
+            m = np.ascontiguousarray(level.mon[n][cells])
+            m[:k] *= 3.0
+            keep.append(m)
+            nb.mon = _abi.dptr(m)
+        else:
+            M = np.ascontiguousarray(level.M[n][cells])
+            x = np.ascontiguousarray(level.com[n][cells])
+            M[:k] *= 3.0
+            x[:k] *= 3.0
+            keep += [M, x]
+            nb.M, nb.com = _abi.dptr(M), _abi.dptr(x)
+    keep.append(level)
+    return sg, keep
+
+
+def zeroed_level(level, spec, slot=13):
+    """A copy of the level in which every neighbour named in spec holds zeros
+    in the cells its list leaves out (multipoles and centres of mass of
+    refined neighbours, monopoles of leaf neighbours): what the reference's
+    staging builds from a partial halo."""
+    import copy
+    lv = copy.deepcopy(level)
+    for raw, cells in spec.items():
+        n = int(lv.neighbors[slot, raw])
+        absent = np.ones(512, bool)
+        absent[np.asarray(cells)] = False
+        if lv.kind[n] == _abi.SLOT_LEAF:
+            lv.mon[n][absent] = 0.0
+        else:
+            lv.M[n][absent] = 0.0
+            lv.com[n][absent] = 0.0
+    return lv
+
+
+def partial_specs(name):
+    """Partial-halo patterns over a case's level: every cell of two neighbours
+    in shuffled order; subsets of them; the subsets preceded by stale repeats
+    of some of their cells (pass stale=stale_prefix(spec) to partial_subgrid)."""
+    lv = build_level(name)
+    rng = np.random.default_rng(hash_name(name) + 7)
+    refined = [r for r in range(27) if r != 13 and lv.neighbors[13, r] >= 0
+               and lv.kind[lv.neighbors[13, r]] == _abi.SLOT_NONLEAF]
+    leaves = [r for r in range(27) if r != 13 and lv.neighbors[13, r] >= 0
+              and lv.kind[lv.neighbors[13, r]] == _abi.SLOT_LEAF]
+    pick = (refined[:1] + leaves[:1]) if leaves and refined else (refined or leaves)[:2]
+    full = {r: rng.permutation(512).astype(np.int32) for r in pick}
+    sub = {r: rng.permutation(512)[: 150 + 100 * i].astype(np.int32) for i, r in enumerate(pick)}
+    rep = {r: np.concatenate([c[::3], c]).astype(np.int32) for r, c in sub.items()}
+    return {"all_shuffled": full, "subset": sub, "repeats": rep}
+
+
+def stale_prefix(spec_name, spec):
+    """Leading stale entries of the 'repeats' pattern (none for the others)."""
+    if spec_name != "repeats":
+        return None
+    return {r: int(c.size - (c.size * 3) // 4) for r, c in spec.items()}

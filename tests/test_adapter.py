@@ -29,11 +29,15 @@ def test_adapter_compiles_against_octotiger_shaped_types(tmp_path):
     assert os.path.exists(build(tmp_path))
 
 
-def write_case(path, name, t):
+def write_case(path, name, t, spec=None):
+    """The case's 27 slots; the neighbours in spec (raw direction -> cell
+    list) hand over only those cells, listed by the stand-in grid's
+    get_ilist_n_bnd."""
     lv = fmm_cases.build_level(name)
     kind = dict(fmm_cases.CASES)[name]
     cmp_ = any(lv.neighbors[13, r] >= 0 and lv.kind[lv.neighbors[13, r]] == _abi.SLOT_NONLEAF
                for r in range(27) if r != 13)
+    spec = spec or {}
     with open(path, "wb") as f:
         np.array([1 if kind == "monopole" else 0, t, 1 if kind == "root" else 0, 1 if cmp_ else 0],
                  np.int32).tofile(f)
@@ -41,24 +45,35 @@ def write_case(path, name, t):
         for raw in range(27):
             n = int(lv.neighbors[13, raw])
             k = -1 if n < 0 else int(lv.kind[n])
-            np.array([k], np.int32).tofile(f)
+            cells = np.asarray(spec.get(raw, np.arange(512)), np.int32)
+            np.array([k, cells.size], np.int32).tofile(f)
+            cells.tofile(f)
             s = max(n, 0)
             for a in (lv.M[s], lv.com[s], lv.mon[s]):
-                np.ascontiguousarray(a, np.float64).tofile(f)
+                np.ascontiguousarray(a[cells], np.float64).tofile(f)
     return lv, kind, cmp_
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["mixed_halo", "c1_root", "leaf_p2m"])
-def test_adapter_on_gpu_matches_oracle(tmp_path, name):
+@pytest.mark.parametrize("name,pattern", [("mixed_halo", None), ("c1_root", None), ("leaf_p2m", None),
+                                          ("mixed_halo", "subset"), ("c1_nonroot", "repeats"),
+                                          ("leaf_p2m", "subset")])
+def test_adapter_on_gpu_matches_oracle(tmp_path, name, pattern):
+    """Octo-Tiger-shaped calls through the adapter, full halos and partial
+    halos (the adapter reads the cells of a partial halo from the grid's
+    get_ilist_n_bnd list, as update_input does), against the oracle."""
     exe = build(tmp_path)
     inp, out = os.path.join(str(tmp_path), "in.bin"), os.path.join(str(tmp_path), "out.bin")
-    lv, kind, cmp_ = write_case(inp, name, _abi.RHO)
-    print(f"adapter case {name}", flush=True)
+    spec = fmm_cases.partial_specs(name)[pattern] if pattern else None
+    lv, kind, cmp_ = write_case(inp, name, _abi.RHO, spec)
+    print(f"adapter case {name} partial {pattern}", flush=True)
     subprocess.run([exe, inp, out], check=True, timeout=120)
     res = np.fromfile(out, np.float64)
     L, Lc = res[: 512 * 20].reshape(512, 20), res[512 * 20:].reshape(512, 3)
-    sg, keep, _, _ = fmm_cases.subgrid(name, _abi.RHO)
+    if spec:
+        sg, keep = fmm_cases.partial_subgrid(lv, spec)
+    else:
+        sg, keep, _, _ = fmm_cases.subgrid(name, _abi.RHO)
     if kind == "monopole":
         Lr, Lcr = oracle.monopole_interactions(sg, cmp_)
     else:

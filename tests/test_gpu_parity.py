@@ -300,3 +300,37 @@ def test_dropin_checks_finiteness_when_asked():
         c.multipole_interactions(sg, L2, Lc)
     assert e.value.status == _abi.ERR_NONFINITE and not L2.any()
     c.close()
+
+
+def _gpu_case(ctx, kind, lv, sg):
+    cmp_ = any(lv.neighbors[13, r] >= 0 and lv.kind[lv.neighbors[13, r]] == _abi.SLOT_NONLEAF
+               for r in range(27) if r != 13)
+    L, Lc = np.zeros((512, 20)), np.zeros((512, 3))
+    if kind == "monopole":
+        ctx.monopole_interactions(sg, cmp_, L, Lc)
+        Lr, Lcr = oracle.monopole_interactions(sg, cmp_)
+    else:
+        ctx.multipole_interactions(sg, L, Lc)
+        Lr, Lcr = oracle.multipole_interactions(sg)
+    return L, Lc, Lr, Lcr
+
+
+@pytest.mark.parametrize("name", ["c1_nonroot", "mixed_halo", "leaf_p2p", "leaf_p2m"])
+@pytest.mark.parametrize("pattern", ["all_shuffled", "subset", "repeats"])
+def test_partial_halo_scattered_on_device(ctx, name, pattern):
+    """Partial halos handed over in grid::get_ilist_n_bnd order are scattered
+    into their slots on the device: the result equals the full-halo call on a
+    level whose left-out cells hold zeros, bit for bit (every cell in shuffled
+    order: the full-halo result itself; repeated cells: the later entry wins),
+    and the oracle's to 1e-12."""
+    lv = fmm_cases.build_level(name)
+    kind = dict(fmm_cases.CASES)[name]
+    spec = fmm_cases.partial_specs(name)[pattern]
+    print(f"partial halo {name} {pattern}", flush=True)
+    sg, keep = fmm_cases.partial_subgrid(lv, spec, stale=fmm_cases.stale_prefix(pattern, spec))
+    L, Lc, Lr, Lcr = _gpu_case(ctx, kind, lv, sg)
+    full = lv if pattern == "all_shuffled" else fmm_cases.zeroed_level(lv, spec)
+    sgf, keepf = workloads.subgrid_view(full, 13)
+    Lf, Lcf, _, _ = _gpu_case(ctx, kind, full, sgf)
+    assert np.array_equal(L, Lf) and np.array_equal(Lc, Lcf)
+    assert normwise(L, Lr) <= TOL and normwise(Lc, Lcr) <= TOL

@@ -388,3 +388,34 @@ def test_vector_reference_baseline_matches_oracle(t):
     assert normwise(Ls[tg], Lr[tg]) <= 1e-12
     if t == _abi.RHO:
         assert normwise(Lcs[tg], Lcr[tg]) <= 1e-12
+
+
+def _run_case(kind, lv, sg, t, use_ref=False):
+    cmp_ = any(lv.neighbors[13, r] >= 0 and lv.kind[lv.neighbors[13, r]] == _abi.SLOT_NONLEAF
+               for r in range(27) if r != 13)
+    if kind == "monopole":
+        return oracle.monopole_interactions(sg, cmp_, use_ref=use_ref)
+    return oracle.multipole_interactions(sg, use_ref=use_ref)
+
+
+@pytest.mark.parametrize("name", ["c1_nonroot", "mixed_halo", "leaf_p2p", "leaf_p2m"])
+@pytest.mark.parametrize("pattern", ["all_shuffled", "subset", "repeats"])
+def test_partial_halo_equals_full_halo_with_absent_cells_zeroed(name, pattern):
+    """A partial halo (cells handed over in a list, as grid::get_ilist_n_bnd
+    orders them) stages exactly what a full halo whose left-out cells hold
+    zeros stages: every cell in shuffled order gives the full-halo result bit
+    for bit, a subset gives the zeroed full halo's, and with repeated cells
+    the later entry wins (update_input overwrites in list order)."""
+    lv = fmm_cases.build_level(name)
+    kind = dict(fmm_cases.CASES)[name]
+    spec = fmm_cases.partial_specs(name)[pattern]
+    sg, keep = fmm_cases.partial_subgrid(lv, spec, stale=fmm_cases.stale_prefix(pattern, spec))
+    full = lv if pattern == "all_shuffled" else fmm_cases.zeroed_level(lv, spec)
+    sgf, keepf = workloads.subgrid_view(full, 13)
+    for use_ref in ([False, True] if oracle.ref_available() else [False]):
+        L, Lc = _run_case(kind, lv, sg, _abi.RHO, use_ref)
+        Lf, Lcf = _run_case(kind, full, sgf, _abi.RHO, use_ref)
+        assert np.array_equal(L, Lf) and np.array_equal(Lc, Lcf)
+        if pattern != "all_shuffled":
+            L0, _ = _run_case(kind, lv, workloads.subgrid_view(lv, 13)[0], _abi.RHO, use_ref)
+            assert not np.array_equal(L, L0)      # the left-out cells mattered
