@@ -191,10 +191,16 @@ typedef struct octofmm_level {
 #define OCTOFMM_PLAN_AUTOTUNE 2    /* time the kernel variants once and keep the fastest */
 /* The environment variable OCTOFMM_VARIANT, when it names a variant of the
  * multipole-target kernel (lane mapping x 100 + waves per workgroup: 8, 4, 2,
- * 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702, 708;
- * mixed precision: 8, 4, 2), fixes that variant for every plan created while
- * it is set, in place of the default or the autotuner's choice.  All variants
- * give the same results to rounding; it exists for measurement. */
+ * 304, 302, 504, 704, 702, 708; mixed precision: 8, 4, 2), fixes that variant
+ * for every plan created while it is set, in place of the default or the
+ * autotuner's choice.  All variants give the same results to rounding; it
+ * exists for measurement.
+ * With OCTOFMM_PLAN_AUTOTUNE the choice is cached by level shape (device
+ * architecture and CU count, solve type, precision, opening angle, counts of
+ * multipole-target items with and without leaf neighbours, fused downward pass
+ * or not): a plan of a shape seen before in the process takes the cached
+ * variant without timing.  When the environment variable OCTOFMM_TUNE_CACHE
+ * names a file, the cache is read from it and each new choice appended to it. */
 /* The level is the root sub-grid alone (nslots = ntargets = 1, non-leaf):
  * every pair of its cells except self and near offsets, the root stencil of
  * octotiger/multipole_interactions/kernel/kokkos_kernel.hpp:324-471, 637-764.

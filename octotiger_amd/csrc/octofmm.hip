@@ -26,7 +26,9 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -120,10 +122,11 @@ struct LaneGeom {
     int rec;          // record index inside the slot
 };
 
-// wave w (0..7 across the sub-grid) owns coarse block ((w>>2)&1, (w>>1)&1, w&1)
-__device__ __forceinline__ LaneGeom lane_geom(int w) {
+// wave w (0..7 across the sub-grid) owns coarse block ((w>>2)&1, (w>>1)&1, w&1);
+// t is the thread's index in the workgroup
+__device__ __forceinline__ LaneGeom lane_geom(int w, int t = threadIdx.x) {
     LaneGeom g;
-    const int l = threadIdx.x & 63;
+    const int l = t & 63;
     const int c = l >> 3;
     g.k = l & 7;
     g.Cx = ((w >> 2) & 1) * 2 + ((c >> 2) & 1);
@@ -481,111 +484,8 @@ __device__ __forceinline__ void lds_wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// LDS stage of one wave: 8 coarse-cell groups of 8 partner records
-// (12 double2 each), each group padded by one double2 so the 8 broadcast
-// addresses of a read fall into different banks.
-constexpr int kStageGroup = 8 * 12 + 1;
-constexpr int kStageWave = 8 * kStageGroup;
-
-// Mapping 2: mapping 0 with the partner records of each coarse offset staged
-// once per wave in LDS and read back as 8-lane broadcasts by the 8 child
-// iterations.  Staging is fully coalesced: the 8 partner coarse cells of the
-// wave each hold their 8 records contiguously (1536 B), and lane l moves
-// 16-byte chunk l of every 1 KiB of that data.
-template <bool RHO>
-__device__ void m2l_item_lds(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind, int w,
-    double2* __restrict__ stage) {
-    const LaneGeom g = lane_geom(w);
-    const int l = threadIdx.x & 63, c = l >> 3;
-    const Tables* __restrict__ tab = a.tab;
-    const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
-    const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
-    const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
-    double mc[10];
-#pragma unroll
-    for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
-    double L[20], C[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int i = 0; i < 20; ++i) L[i] = 0.0;
-    const int nd = tab->ndelta;
-    // wave-uniform: the wave's coarse block and the neighbour table row
-    const int wu = __builtin_amdgcn_readfirstlane(w);
-    const int bx = ((wu >> 2) & 1) * 2, by = ((wu >> 1) & 1) * 2, bz = (wu & 1) * 2;
-    const int32_t* nbr_row = a.nbr + (size_t)__builtin_amdgcn_readfirstlane(slot) * 27;
-    // base address of the records of partner coarse cell (block + group g) + delta
-    auto group_base = [&](int gi, int ax, int ay, int az) -> const double2* {
-        const int px = bx + ((gi >> 2) & 1) + ax, py = by + ((gi >> 1) & 1) + ay, pz = bz + (gi & 1) + az;
-        const int raw = (px < 0 ? 0 : (px < 4 ? 1 : 2)) + 3 * (py < 0 ? 0 : (py < 4 ? 1 : 2)) +
-            9 * (pz < 0 ? 0 : (pz < 4 ? 1 : 2));
-        const int ns = nbr_row[raw];
-        const int cb = ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
-        return reinterpret_cast<const double2*>(a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles);
-    };
-    // chunk i*64 + l of the 768 staged chunks: group q / 96, offset q % 96
-    double2 pre[12];
-    auto fetch = [&](int di) {
-        const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
-        const double2* gb[8];
-#pragma unroll
-        for (int gi = 0; gi < 8; ++gi) gb[gi] = group_base(gi, ax, ay, az);
-#pragma unroll
-        for (int i = 0; i < 12; ++i) {
-            const int q = i * 64 + l;
-            const int glo = (i * 64) / 96, ghi = (i * 64 + 63) / 96;
-            const int gq = q >= (glo + 1) * 96 ? ghi : glo;
-            pre[i] = (gq == glo ? gb[glo] : gb[ghi])[q - gq * 96];
-        }
-    };
-    auto far_bits = [&](int di) -> unsigned {
-        unsigned fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
-        int ns, raw;
-        partner_base(g, tab->delta[di][0], tab->delta[di][1], tab->delta[di][2], s_nbr, ns, raw);
-        return ns < 0 ? 0u : fb;
-    };
-    const double2* group = stage + c * kStageGroup;
-    fetch(0);
-    for (int di = 0; di < nd; ++di) {
-#pragma unroll
-        for (int i = 0; i < 12; ++i) {
-            const int q = i * 64 + l;
-            const int gq = q / 96;
-            stage[gq * kStageGroup + (q - gq * 96)] = pre[i];
-        }
-        lds_wave_sync();
-        const unsigned fb = far_bits(di);
-        if (di + 1 < nd) fetch(di + 1);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            if (fb & (1u << j)) {
-                Rec cur;
-#pragma unroll
-                for (int i = 0; i < 12; ++i) cur.v[i] = group[j * 12 + i];
-                m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
-            }
-        }
-        lds_wave_sync();
-    }
-    if (flags & kItemNear) {
-        for (int di = 0; di < nd; ++di) {
-            const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
-            unsigned nb = (unsigned)(tab->nearbits[di] >> (8 * g.k)) & 0xffu;
-            int ns, raw;
-            const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
-            if (ns < 0 || s_kind[raw] != OCTOFMM_SLOT_LEAF) nb = 0;
-            const double* pb2 = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
-#pragma unroll 1
-            for (int j = 0; j < 8; ++j) {
-                if (nb & (1u << j)) {
-                    const double* p = pb2 + j * kRecDoubles;
-                    m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, inv_mc0, mc, L, C);
-                }
-            }
-        }
-    }
-    finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
-}
-
-// Mapping 3: mapping 2 with the staging done by direct global->LDS loads
+// Mapping 3: the partner records of each coarse offset staged once per wave
+// in LDS by direct global->LDS loads
 // (global_load_lds_dwordx4): the 64 partner records a wave needs for one
 // coarse offset (8 partner coarse cells x 8 children, 12 KiB) go from L2
 // into the wave's LDS image without passing through registers, 16 bytes per
@@ -614,10 +514,10 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     int w, double2* __restrict__ s_stage, int sl = 0) {
     // this wave's image (wave-uniform address)
     double2* stage = s_stage + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * kDirWave;
-    const LaneGeom g = lane_geom(w);
     const int l = threadIdx.x & 63, c = l >> 3;
     const auto* tab = as_const(a.tab);
-    const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const double* trec = a.rec + ((size_t)slot * kCells + lane_geom(wu).rec) * kRecDoubles;
     const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
     const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
     double mc[10];
@@ -627,15 +527,17 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
 #pragma unroll
     for (int i = 0; i < 20; ++i) L[i] = 0.0;
     const int nd = tab->ndelta;
-    const int wu = __builtin_amdgcn_readfirstlane(w);
     const int bx = ((wu >> 2) & 1) * 2, by = ((wu >> 1) & 1) * 2, bz = (wu & 1) * 2;
     const auto* nbr_row = as_const(a.nbr + (size_t)__builtin_amdgcn_readfirstlane(slot) * 27);
-    auto group_base = [&](int gi, int ax, int ay, int az) -> const double2* {
+    // partner coarse cell of group gi (coarse cell gi of the wave's block) at
+    // offset a; empty directions are marked in bit gi of `empty`
+    auto group_base = [&](int gi, int ax, int ay, int az, unsigned& empty) -> const double2* {
         const int px = bx + ((gi >> 2) & 1) + ax, py = by + ((gi >> 1) & 1) + ay, pz = bz + (gi & 1) + az;
         const int raw = (px < 0 ? 0 : (px < 4 ? 1 : 2)) + 3 * (py < 0 ? 0 : (py < 4 ? 1 : 2)) +
             9 * (pz < 0 ? 0 : (pz < 4 ? 1 : 2));
         const int ns = nbr_row[raw];
         const int cb = ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
+        if (ns < 0) empty |= 1u << gi;
         return reinterpret_cast<const double2*>(a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles);
     };
     // Group c (8 records = 96 chunks) is loaded by one full instruction
@@ -647,12 +549,16 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     // neighbour-slot loads in flight together (one wait, taken while the
     // current image is still arriving, instead of one wait per group).
     const char* nbase[8];
+    unsigned empty_next = 0;   // groups of the offset bases() last formed whose partner slot is empty
     auto bases = [&](int di) {
         const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+        unsigned empty = 0;
 #pragma unroll
-        for (int gi = 0; gi < 8; ++gi) nbase[gi] = reinterpret_cast<const char*>(group_base(gi, ax, ay, az));
+        for (int gi = 0; gi < 8; ++gi)
+            nbase[gi] = reinterpret_cast<const char*>(group_base(gi, ax, ay, az, empty));
         asm volatile("" ::"s"(nbase[0]), "s"(nbase[1]), "s"(nbase[2]), "s"(nbase[3]), "s"(nbase[4]),
                      "s"(nbase[5]), "s"(nbase[6]), "s"(nbase[7]));
+        empty_next = empty;
     };
     // The full-wave loads of all groups come first and the half-wave loads
     // follow in one branch of their own: with a full and a half load of one
@@ -669,11 +575,11 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
                 __builtin_amdgcn_global_load_lds(nbase[gi] + 1024 + loff, (lds_ptr)(stage + gi * kDirGroup + 64), 16, 0, 0);
         }
     };
-    auto far_bits = [&](int di) -> unsigned {
-        unsigned fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
-        int ns, raw;
-        partner_base(g, tab->delta[di][0], tab->delta[di][1], tab->delta[di][2], s_nbr, ns, raw);
-        return ns < 0 ? 0u : fb;
+    // the lane's far children at offset di: none when its partner coarse
+    // cell (group c) lies in an empty direction
+    auto far_bits = [&](int di, unsigned empty) -> unsigned {
+        const unsigned fb = (unsigned)(tab->farbits[di] >> (8 * (l & 7))) & 0xffu;
+        return ((empty >> c) & 1u) ? 0u : fb;
     };
     const double2* group = stage + c * kDirGroup;
     // only the offsets with a far pair are staged (delta 0 is all near / self)
@@ -684,7 +590,7 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
         issue();
     }
     for (int q = q0; q < nfd; q += SPLIT) {
-        const unsigned fb = far_bits(tab->far_delta[q]);
+        const unsigned fb = far_bits(tab->far_delta[q], empty_next);
         if (q + SPLIT < nfd) bases(tab->far_delta[q + SPLIT]);
         __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): this wave's image is in LDS
         __builtin_amdgcn_wave_barrier();
@@ -700,6 +606,12 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
         lds_wave_sync();   // every read of the image done before it is refilled
         if (q + SPLIT < nfd) issue();
     }
+    // The lane's geometry again, from an opaque copy of the thread index: the
+    // far loop needs none of it, and formed here it is not held in registers
+    // across that loop (where the RHO kernels run at the register limit).
+    int t2;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(t2) : "v"((int)threadIdx.x));
+    const LaneGeom g = lane_geom(wu, t2);
     if (flags & kItemNear) {
         for (int di = q0; di < nd; di += SPLIT) {
             const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
@@ -737,221 +649,6 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             C[0] += po[20];
             C[1] += po[21];
             C[2] += po[22];
-        }
-    }
-    finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
-}
-
-// Mapping 4: mapping 3 with the image split by child halves and double
-// buffered.  Buffer h holds children 4h..4h+3 of the 8 partner coarse cells
-// (8 groups of 48 chunks + 1 pad chunk); while the wave computes with one
-// half, the other half is being refilled, so a refill's latency overlaps
-// four pairs of arithmetic instead of stalling the wave.
-constexpr int kHalfGroup = 49;                       // 48 chunks + 1 pad
-constexpr int kHalfWave = 8 * kHalfGroup;            // 392 double2
-
-// One direct global->LDS load of 16 bytes per active lane: lane i writes LDS
-// bytes [lds + 16 i, lds + 16 i + 16) from gbase + voff.  Issued as inline
-// assembly so the compiler does not track it: the caller orders it with
-// explicit waits (vmcnt before reading the bytes, LDS reads drained before
-// refilling them).
-__device__ __forceinline__ void lds_dma16(const void* gbase, unsigned voff, unsigned lds) {
-    asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1"
-                 :
-                 : "v"(voff), "s"(gbase), "s"(lds)
-                 : "memory", "m0");
-}
-
-__device__ __forceinline__ unsigned lds_offset(const void* p) {
-    return (unsigned)(size_t)(lds_ptr)p;
-}
-
-template <bool RHO>
-__device__ void m2l_item_direct2(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind,
-    int w, double2* __restrict__ s_half0, double2* __restrict__ s_half1) {
-    // the two halves live in two distinct LDS arrays, so reads of one half
-    // need not wait for the refill of the other
-    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    double2* const half[2] = {s_half0 + wv * kHalfWave, s_half1 + wv * kHalfWave};
-    const LaneGeom g = lane_geom(w);
-    const int l = threadIdx.x & 63, c = l >> 3;
-    const auto* tab = as_const(a.tab);
-    const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
-    const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
-    const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
-    double mc[10];
-#pragma unroll
-    for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
-    double L[20], C[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int i = 0; i < 20; ++i) L[i] = 0.0;
-    const int nd = tab->ndelta;
-    const int wu = __builtin_amdgcn_readfirstlane(w);
-    const int bx = ((wu >> 2) & 1) * 2, by = ((wu >> 1) & 1) * 2, bz = (wu & 1) * 2;
-    const auto* nbr_row = as_const(a.nbr + (size_t)__builtin_amdgcn_readfirstlane(slot) * 27);
-    auto group_base = [&](int gi, int ax, int ay, int az) -> const char* {
-        const int px = bx + ((gi >> 2) & 1) + ax, py = by + ((gi >> 1) & 1) + ay, pz = bz + (gi & 1) + az;
-        const int raw = (px < 0 ? 0 : (px < 4 ? 1 : 2)) + 3 * (py < 0 ? 0 : (py < 4 ? 1 : 2)) +
-            9 * (pz < 0 ? 0 : (pz < 4 ? 1 : 2));
-        const int ns = nbr_row[raw];
-        const int cb = ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
-        return reinterpret_cast<const char*>(a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles);
-    };
-    // 8 loads on the lower 48 lanes: chunk l of children 4h..4h+3 of group gi.
-    // The 8 partner bases of an offset are formed once, all their scalar
-    // loads in flight together, and serve both halves.
-    const unsigned loff = (unsigned)l * 16u;
-    const char* nbase[8];
-    auto bases = [&](int di) {
-        const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
-#pragma unroll
-        for (int gi = 0; gi < 8; ++gi) nbase[gi] = group_base(gi, ax, ay, az);
-        // formed here, so the 8 neighbour-slot loads share one wait instead of
-        // each waiting next to its refill
-        asm volatile("" ::"s"(nbase[0]), "s"(nbase[1]), "s"(nbase[2]), "s"(nbase[3]), "s"(nbase[4]),
-                     "s"(nbase[5]), "s"(nbase[6]), "s"(nbase[7]));
-    };
-    auto issue = [&](int h) {
-        double2* buf = half[h];
-        if (l < 48) {
-#pragma unroll
-            for (int gi = 0; gi < 8; ++gi) {
-                const unsigned dst = (unsigned)__builtin_amdgcn_readfirstlane((int)lds_offset(buf + gi * kHalfGroup));
-                lds_dma16(nbase[gi] + h * 48 * 16, loff, dst);
-            }
-        }
-    };
-    auto far_bits = [&](int di) -> unsigned {
-        unsigned fb = (unsigned)(tab->farbits[di] >> (8 * g.k)) & 0xffu;
-        int ns, raw;
-        partner_base(g, tab->delta[di][0], tab->delta[di][1], tab->delta[di][2], s_nbr, ns, raw);
-        return ns < 0 ? 0u : fb;
-    };
-    // the target's own loads complete before the first refill, so no later
-    // wait for them can drain the refills in flight
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0)
-    asm volatile("" ::: "memory");
-    // only the offsets with a far pair are staged (delta 0 is all near / self)
-    const int nfd = tab->nfar_delta;
-    if (nfd > 0) {
-        bases(tab->far_delta[0]);
-        issue(0);
-        issue(1);
-    }
-    for (int q = 0; q < nfd; ++q) {
-        const unsigned fb = far_bits(tab->far_delta[q]);
-        const bool more = q + 1 < nfd;
-        if (more) bases(tab->far_delta[q + 1]);
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            // this half's 8 loads are done; the other half's 8 may still fly
-            __builtin_amdgcn_s_waitcnt(0x3F78);   // vmcnt(8)
-            __builtin_amdgcn_wave_barrier();
-            const double2* group = half[h] + c * kHalfGroup;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                if (fb & (1u << (4 * h + r))) {
-                    Rec cur;
-#pragma unroll
-                    for (int i = 0; i < 12; ++i) cur.v[i] = group[r * 12 + i];
-                    m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
-                }
-            }
-            lds_wave_sync();   // every read of this half done before it is refilled
-            if (more) {
-                issue(h);
-            } else {
-                __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): nothing more in flight to wait behind
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
-    }
-    if (flags & kItemNear) {
-        for (int di = 0; di < nd; ++di) {
-            const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
-            unsigned nb = (unsigned)(tab->nearbits[di] >> (8 * g.k)) & 0xffu;
-            int ns, raw;
-            const int cb = partner_base(g, ax, ay, az, s_nbr, ns, raw);
-            if (ns < 0 || s_kind[raw] != OCTOFMM_SLOT_LEAF) nb = 0;
-            const double* pb2 = a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
-#pragma unroll 1
-            for (int j = 0; j < 8; ++j) {
-                if (nb & (1u << j)) {
-                    const double* p = pb2 + j * kRecDoubles;
-                    m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, inv_mc0, mc, L, C);
-                }
-            }
-        }
-    }
-    finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
-}
-
-// Mapping 1: wave = child class k over the 64 coarse cells of the sub-grid;
-// the (delta, j) partner list is the same for every lane.
-template <bool RHO>
-__device__ void m2l_item_classwave(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind,
-    int k) {
-    const int l = threadIdx.x & 63;
-    LaneGeom g;
-    g.k = k;
-    g.Cx = l >> 4;
-    g.Cy = (l >> 2) & 3;
-    g.Cz = l & 3;
-    const int fx = 2 * g.Cx + ((k >> 2) & 1), fy = 2 * g.Cy + ((k >> 1) & 1), fz = 2 * g.Cz + (k & 1);
-    g.cell = fx * 64 + fy * 8 + fz;
-    g.rec = (g.Cx * 16 + g.Cy * 4 + g.Cz) * 8 + k;
-    const Tables* __restrict__ tab = a.tab;
-    const double* trec = a.rec + ((size_t)slot * kCells + g.rec) * kRecDoubles;
-    const double X0 = trec[kRecY], X1 = trec[kRecY + 1], X2 = trec[kRecY + 2];
-    const double inv_mc0 = RHO ? 1.0 / trec[0] : 0.0;
-    double mc[10];
-#pragma unroll
-    for (int i = 0; i < 10; ++i) mc[i] = RHO ? trec[10 + i] : 0.0;
-    double L[20], C[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int i = 0; i < 20; ++i) L[i] = 0.0;
-    auto base_of = [&](int di, bool& ok, int& raw) -> const double* {
-        int ns;
-        const int cb = partner_base(g, tab->delta[di][0], tab->delta[di][1], tab->delta[di][2], s_nbr, ns, raw);
-        ok = ns >= 0;
-        return a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles;
-    };
-    {
-        const uint16_t* list = tab->farlist[k];
-        const int n = tab->nfar_k[k];
-        int e = list[0], di = e >> 3, raw;
-        bool okn;
-        const double* bnx = base_of(di, okn, raw);
-        Rec nxt;
-        load_rec(bnx + (e & 7) * kRecDoubles, nxt);
-#pragma unroll 2
-        for (int q = 0; q < n; ++q) {
-            const Rec cur = nxt;
-            const bool okc = okn;
-            if (q + 1 < n) {
-                const int e1 = list[q + 1], di1 = e1 >> 3;
-                if (di1 != di) {
-                    di = di1;
-                    bnx = base_of(di1, okn, raw);
-                }
-                load_rec(bnx + (e1 & 7) * kRecDoubles, nxt);
-            }
-            if (okc) m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
-        }
-    }
-    if (flags & kItemNear) {
-        const uint16_t* list = tab->nearlist[k];
-        const int n = tab->nnear_k[k];
-        for (int q = 0; q < n; ++q) {
-            const int e = list[q];
-            bool ok;
-            int raw;
-            const double* pb = base_of(e >> 3, ok, raw);
-            if (ok && s_kind[raw] == OCTOFMM_SLOT_LEAF) {
-                const double* p = pb + (e & 7) * kRecDoubles;
-                m2l_near<RHO>(p[kRecMon], p[kRecY], p[kRecY + 1], p[kRecY + 2], X0, X1, X2, inv_mc0, mc, L, C);
-            }
         }
     }
     finish_cell<RHO>(a, slot, g.cell, L, 20, C, true, false);
@@ -1192,12 +889,11 @@ void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, const Tables& 
 // One target sub-grid is split over 8 / WAVES workgroups of WAVES waves.  The
 // workgroup index is first spread over the XCDs (contiguous runs per XCD),
 // so the parts of one sub-grid and its neighbours share an L2.
-template <bool RHO, int WAVES, int MAPPING, bool MIXED = false>
+template <bool RHO, int WAVES, bool MIXED = false>
 __global__ __launch_bounds__(64 * WAVES) void level_kernel(LevelArgs a) {
     constexpr int kParts = 8 / WAVES;
     __shared__ int s_nbr[27];
     __shared__ int s_kind[27];
-    __shared__ double2 s_stage[MAPPING == 2 ? WAVES * kStageWave : 1];
     const int v = xcd_item(blockIdx.x, gridDim.x);
     const int item = v / kParts;
     const int w = (v % kParts) * WAVES + (threadIdx.x >> 6);
@@ -1210,22 +906,16 @@ __global__ __launch_bounds__(64 * WAVES) void level_kernel(LevelArgs a) {
     __syncthreads();
     if (it.y & kItemLeaf)
         return;   // leaf items run in leaf_kernel
-    if (MAPPING == 1)
-        m2l_item_classwave<RHO>(a, it.x, it.y, s_nbr, s_kind, w);
-    else if (MAPPING == 2)
-        m2l_item_lds<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage + (threadIdx.x >> 6) * kStageWave);
-    else
-        m2l_item<RHO, MIXED>(a, it.x, it.y, s_nbr, s_kind, w);
+    m2l_item<RHO, MIXED>(a, it.x, it.y, s_nbr, s_kind, w);
 }
 
-// Mapping-3/4 kernel (direct global->LDS staging); one wave per 2x2x2 coarse block.
-template <bool RHO, int WAVES, int MAPPING>
+// Mapping-3 kernel (direct global->LDS staging); one wave per 2x2x2 coarse block.
+template <bool RHO, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3))) void level_kernel_direct(LevelArgs a) {
     constexpr int kParts = 8 / WAVES;
     __shared__ int s_nbr[27];
     __shared__ int s_kind[27];
-    __shared__ double2 s_stage[MAPPING == 4 ? WAVES * kHalfWave : WAVES * kDirWave];
-    __shared__ double2 s_stage1[MAPPING == 4 ? WAVES * kHalfWave : 1];
+    __shared__ double2 s_stage[WAVES * kDirWave];
     const int v = xcd_item(blockIdx.x, gridDim.x);
     const int item = v / kParts;
     const int w = (v % kParts) * WAVES + (threadIdx.x >> 6);
@@ -1237,10 +927,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
     }
     __syncthreads();
     if (it.y & kItemLeaf) return;
-    if (MAPPING == 4)
-        m2l_item_direct2<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage, s_stage1);
-    else
-        m2l_item_direct<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage);
+    m2l_item_direct<RHO>(a, it.x, it.y, s_nbr, s_kind, w, s_stage);
 }
 
 // Mapping-7 kernel: one workgroup of SPLIT waves per 2x2x2 coarse block, the
@@ -1270,22 +957,17 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
     const dim3 grid(nitems * parts), block(64 * waves);
     if (mixed) {
         switch (variant) {
-        case 8: hipLaunchKernelGGL((level_kernel<RHO, 8, 0, true>), grid, block, 0, st, la); break;
-        case 4: hipLaunchKernelGGL((level_kernel<RHO, 4, 0, true>), grid, block, 0, st, la); break;
-        default: hipLaunchKernelGGL((level_kernel<RHO, 2, 0, true>), grid, block, 0, st, la); break;
+        case 8: hipLaunchKernelGGL((level_kernel<RHO, 8, true>), grid, block, 0, st, la); break;
+        case 4: hipLaunchKernelGGL((level_kernel<RHO, 4, true>), grid, block, 0, st, la); break;
+        default: hipLaunchKernelGGL((level_kernel<RHO, 2, true>), grid, block, 0, st, la); break;
         }
         return;
     }
     switch (variant) {
-    case 8: hipLaunchKernelGGL((level_kernel<RHO, 8, 0>), grid, block, 0, st, la); break;
-    case 4: hipLaunchKernelGGL((level_kernel<RHO, 4, 0>), grid, block, 0, st, la); break;
-    case 2: hipLaunchKernelGGL((level_kernel<RHO, 2, 0>), grid, block, 0, st, la); break;
-    case 108: hipLaunchKernelGGL((level_kernel<RHO, 8, 1>), grid, block, 0, st, la); break;
-    case 104: hipLaunchKernelGGL((level_kernel<RHO, 4, 1>), grid, block, 0, st, la); break;
-    case 102: hipLaunchKernelGGL((level_kernel<RHO, 2, 1>), grid, block, 0, st, la); break;
-    case 304: hipLaunchKernelGGL((level_kernel_direct<RHO, 4, 3>), grid, block, 0, st, la); break;
-    case 302: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 3>), grid, block, 0, st, la); break;
-    case 404: hipLaunchKernelGGL((level_kernel_direct<RHO, 4, 4>), grid, block, 0, st, la); break;
+    case 4: hipLaunchKernelGGL((level_kernel<RHO, 4>), grid, block, 0, st, la); break;
+    case 2: hipLaunchKernelGGL((level_kernel<RHO, 2>), grid, block, 0, st, la); break;
+    case 304: hipLaunchKernelGGL((level_kernel_direct<RHO, 4>), grid, block, 0, st, la); break;
+    case 302: hipLaunchKernelGGL((level_kernel_direct<RHO, 2>), grid, block, 0, st, la); break;
     case 504:
         hipLaunchKernelGGL((level_kernel_split<RHO, 4>), dim3(nitems * 8), dim3(256), 0, st, la);
         break;
@@ -1298,10 +980,7 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
     case 708:
         hipLaunchKernelGGL((level_kernel_dsplit<RHO, 8>), dim3(nitems * 8), dim3(512), 0, st, la);
         break;
-    case 402: hipLaunchKernelGGL((level_kernel_direct<RHO, 2, 4>), grid, block, 0, st, la); break;
-    case 208: hipLaunchKernelGGL((level_kernel<RHO, 8, 2>), grid, block, 0, st, la); break;
-    case 204: hipLaunchKernelGGL((level_kernel<RHO, 4, 2>), grid, block, 0, st, la); break;
-    default: hipLaunchKernelGGL((level_kernel<RHO, 2, 2>), grid, block, 0, st, la); break;
+    default: hipLaunchKernelGGL((level_kernel<RHO, 8>), grid, block, 0, st, la); break;
     }
 }
 
@@ -1652,6 +1331,8 @@ struct octofmm_plan {
     int2* d_items = nullptr;
     octofmm_plan_stats stats;
     int variant = 8;           // level-kernel variant (see kVariants)
+    int32_t mixed_far_d2 = 4;
+    std::vector<int2> h_items;     // the work items (host copy, for the tuning key)
     uint64_t* d_mixbits = nullptr;
     bool root = false;             // root sub-grid: root stencil over slot 0
     // set by a whole-tree plan: write final expansions (fused downward pass)
@@ -2101,14 +1782,17 @@ octofmm_status octofmm_destroy(octofmm_context* c) {
     return OCTOFMM_OK;
 }
 
-octofmm_status octofmm_compute_multipole_interactions(octofmm_context* c, const octofmm_subgrid* in,
-    double* L, double* L_c) {
-    if (!c) return OCTOFMM_ERR_INVALID_ARGUMENT;
+}   // extern "C"
+
+// Argument checks of the multipole drop-ins; sets the item flags and the root
+// choice of the call.
+static octofmm_status check_multipole_call(octofmm_context* c, const octofmm_subgrid* in, double* L, double* L_c,
+    int& flags, bool& root) {
     if (!in || !in->M || !in->com || !L) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "null input or output");
     if (in->type != OCTOFMM_RHO && in->type != OCTOFMM_DRHODT) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad type");
     if (in->type == OCTOFMM_RHO && !L_c) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "RHO needs L_c");
-    const bool root = in->use_root_stencil != 0;
-    int flags = 0;
+    root = in->use_root_stencil != 0;
+    flags = 0;
     if (!root)
         for (int f = 0; f < 26; ++f) {
             const octofmm_neighbor& nb = in->neighbors[f];
@@ -2117,7 +1801,17 @@ octofmm_status octofmm_compute_multipole_interactions(octofmm_context* c, const 
                 return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad neighbour %d", f);
             if (nb.kind == OCTOFMM_NEIGHBOR_MONOPOLE && nb.mon) flags |= kItemNear;
         }
-    octofmm_status s = check_finite_inputs(c, in, false);
+    return check_finite_inputs(c, in, false);
+}
+
+extern "C" {
+
+octofmm_status octofmm_compute_multipole_interactions(octofmm_context* c, const octofmm_subgrid* in,
+    double* L, double* L_c) {
+    if (!c) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    int flags = 0;
+    bool root = false;
+    octofmm_status s = check_multipole_call(c, in, L, L_c, flags, root);
     if (s != OCTOFMM_OK) return s;
     HIPC(c, hipSetDevice(c->device));
     (void)hipGetLastError();   // launch checks below report this call's launches only
@@ -2305,6 +1999,79 @@ static octofmm_status plan_run(octofmm_plan* p, hipStream_t st, int mask) {
     return OCTOFMM_OK;
 }
 
+}   // extern "C"
+
+// Kernel variants a plan may run: FP64 plans every variant, mixed-precision
+// plans the mapping-0 forms (8, 4, 2 waves per workgroup).
+static bool variant_allowed(const octofmm_plan* p, int v) {
+    bool known = false;
+    for (int k : kVariants) known = known || k == v;
+    return known && !(p->precision == OCTOFMM_PRECISION_MIXED && v >= 100);
+}
+
+// Tuning cache: the variant the autotuner chose, keyed by level shape (device
+// architecture, solve type, precision, opening angle, counts of multipole-target
+// items with and without a near pass, fused downward pass or not).  A plan of a
+// shape seen before takes the cached choice without timing anything, so plans
+// of one shape run the same kernel, and give the same bits, within a process.
+// When the environment variable OCTOFMM_TUNE_CACHE names a file, the cache is
+// read from it at the first lookup and every new choice is appended to it
+// (one "key variant" line each), so the choices also hold across processes.
+namespace {
+std::mutex g_tune_mu;
+std::vector<std::pair<std::string, int>> g_tune;
+bool g_tune_loaded = false;
+
+std::string tune_key(const octofmm_plan* p) {
+    hipDeviceProp_t prop;
+    const char* arch = hipGetDeviceProperties(&prop, p->ctx->device) == hipSuccess ? prop.gcnArchName : "unknown";
+    int near = 0;
+    for (const int2& it : p->h_items)
+        if (!(it.y & kItemLeaf) && (it.y & kItemNear)) ++near;
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s/cu%d/t%d/p%d/d%d/th%.17g/m%d/n%d/l%d", arch,
+        hipGetDeviceProperties(&prop, p->ctx->device) == hipSuccess ? prop.multiProcessorCount : 0, p->type,
+        p->precision, p->precision == OCTOFMM_PRECISION_MIXED ? p->mixed_far_d2 : 0, p->ctx->theta, p->n_m2l, near,
+        p->l2l ? 1 : 0);
+    for (char* q = buf; *q; ++q)
+        if (*q == ' ') *q = '_';
+    return buf;
+}
+
+// the variant cached for key, or 0
+int tune_lookup(const std::string& key) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if (!g_tune_loaded) {
+        g_tune_loaded = true;
+        if (const char* path = std::getenv("OCTOFMM_TUNE_CACHE")) {
+            if (FILE* f = std::fopen(path, "r")) {
+                char k[256];
+                int v = 0;
+                while (std::fscanf(f, "%255s %d", k, &v) == 2) g_tune.emplace_back(k, v);
+                std::fclose(f);
+            }
+        }
+    }
+    int v = 0;
+    for (const auto& e : g_tune)
+        if (e.first == key) v = e.second;   // the last entry of a key wins
+    return v;
+}
+
+void tune_store(const std::string& key, int v) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune.emplace_back(key, v);
+    if (const char* path = std::getenv("OCTOFMM_TUNE_CACHE")) {
+        if (FILE* f = std::fopen(path, "a")) {
+            std::fprintf(f, "%s %d\n", key.c_str(), v);
+            std::fclose(f);
+        }
+    }
+}
+}   // namespace
+
+extern "C" {
+
 // Times each level-kernel mapping on the live level's multipole-target items
 // (the leaf kernel does not depend on it) and keeps the fastest.  The trial
 // passes write into private scratch outputs, never the caller's L / L_c.
@@ -2344,7 +2111,7 @@ static octofmm_status plan_autotune_run(octofmm_plan* p) {
     float best = 1e30f;
     int best_w = 8;
     for (int w : kVariants) {
-        if (p->precision == OCTOFMM_PRECISION_MIXED && w >= 100) continue;
+        if (!variant_allowed(p, w)) continue;
         p->variant = w;
         s = plan_launch(p, st, 2);   // warm-up
         if (s != OCTOFMM_OK) return s;
@@ -2368,10 +2135,11 @@ static octofmm_status plan_autotune_run(octofmm_plan* p) {
     return OCTOFMM_OK;
 }
 
-octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level* lv, octofmm_plan** out) {
-    if (!c) return OCTOFMM_ERR_INVALID_ARGUMENT;
-    if (!lv || !out) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "null level or plan");
-    *out = nullptr;
+}   // extern "C"
+
+// First half of level plan creation: checks, work items, pair counts, device
+// tables.  The kernel variant and the graph come in level_plan_finish.
+static octofmm_status level_plan_build(octofmm_context* c, const octofmm_level* lv, octofmm_plan** out) {
     if (lv->nslots <= 0 || !lv->neighbors || !lv->kind || !lv->xbase || lv->ntargets < 0 ||
         (lv->ntargets > 0 && !lv->targets))
         return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad level description");
@@ -2571,48 +2339,79 @@ octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level
         }
         (void)per_cell;
     }
+    p->mixed_far_d2 = lv->mixed_far_d2;
+    p->h_items = items;
+    *out = p;
+    return OCTOFMM_OK;
+}
+
+// Second half of plan creation, once the plan's epilogue is wired: the kernel
+// variant (OCTOFMM_VARIANT, else with OCTOFMM_PLAN_AUTOTUNE the tuning cache
+// or a timing of every variant on the live level, else the default) and, with
+// OCTOFMM_PLAN_USE_GRAPH, the captured graph.  Frees the plan on failure.
+static octofmm_status level_plan_finish(octofmm_plan* p, int flags) {
+    octofmm_context* c = p->ctx;
     const char* forced = std::getenv("OCTOFMM_VARIANT");
-    bool forced_ok = false;
-    if (forced)
-        for (int v : kVariants)
-            if (std::atoi(forced) == v && !(p->precision == OCTOFMM_PRECISION_MIXED && v >= 100)) {
-                p->variant = v;
-                forced_ok = true;
-            }
+    const bool forced_ok = forced && variant_allowed(p, std::atoi(forced));
     if (forced_ok) {
-    } else if ((lv->flags & OCTOFMM_PLAN_AUTOTUNE) && p->n_m2l > 0 && !p->root) {
-        // time each workgroup shape on the live level (outputs are scratch
-        // here: the caller zeroes L before a real pass) and keep the fastest
-        octofmm_status ts = plan_autotune(p);
-        if (ts != OCTOFMM_OK) {
-            plan_free(p);
-            return ts;
+        p->variant = std::atoi(forced);
+    } else if ((flags & OCTOFMM_PLAN_AUTOTUNE) && p->n_m2l > 0 && !p->root) {
+        const std::string key = tune_key(p);
+        const int cached = tune_lookup(key);
+        if (cached && variant_allowed(p, cached)) {
+            p->variant = cached;
+        } else {
+            // time each workgroup shape on the live level (outputs are scratch
+            // here: the caller zeroes L before a real pass) and keep the fastest
+            const octofmm_status ts = plan_autotune(p);
+            if (ts != OCTOFMM_OK) {
+                plan_free(p);
+                return ts;
+            }
+            tune_store(key, p->variant);
         }
     }
     p->stats.variant = p->variant;
-    if (lv->flags & OCTOFMM_PLAN_USE_GRAPH) {
+    if (flags & OCTOFMM_PLAN_USE_GRAPH) {
         // the side stream must exist before the capture starts
-        const octofmm_status fs = plan_fork_resources(p);
+        octofmm_status fs = plan_fork_resources(p);
         if (fs != OCTOFMM_OK) {
             plan_free(p);
             return fs;
         }
-    }
-    if (lv->flags & OCTOFMM_PLAN_USE_GRAPH) {
         hipStream_t cs = c->stream;
-        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
-            return cleanup(OCTOFMM_ERR_HIP, "graph capture failed to start");
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            plan_free(p);
+            return fail(c, OCTOFMM_ERR_HIP, "graph capture failed to start");
+        }
         octofmm_status ls = plan_run(p, cs, 7);
         hipGraph_t g = nullptr;
         const hipError_t ee = hipStreamEndCapture(cs, &g);
         if (ls != OCTOFMM_OK || ee != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
-            return cleanup(OCTOFMM_ERR_HIP, "graph capture failed");
+            plan_free(p);
+            return fail(c, OCTOFMM_ERR_HIP, "graph capture failed");
         }
         p->graph = g;
-        if (hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0) != hipSuccess)
-            return cleanup(OCTOFMM_ERR_HIP, "graph instantiation failed");
+        if (hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+            plan_free(p);
+            return fail(c, OCTOFMM_ERR_HIP, "graph instantiation failed");
+        }
     }
+    return OCTOFMM_OK;
+}
+
+extern "C" {
+
+octofmm_status octofmm_level_plan_create(octofmm_context* c, const octofmm_level* lv, octofmm_plan** out) {
+    if (!c) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    if (!lv || !out) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "null level or plan");
+    *out = nullptr;
+    octofmm_plan* p = nullptr;
+    octofmm_status s = level_plan_build(c, lv, &p);
+    if (s != OCTOFMM_OK) return s;
+    s = level_plan_finish(p, lv->flags);
+    if (s != OCTOFMM_OK) return s;
     *out = p;
     return OCTOFMM_OK;
 }
@@ -2942,22 +2741,28 @@ octofmm_status octofmm_tree_plan_create(octofmm_context* c, const octofmm_tree* 
         lv.d_mon = L.d_mon;
         lv.d_L = L.d_L;
         lv.d_L_c = rho ? L.d_L_c : nullptr;
-        const octofmm_status s = octofmm_level_plan_create(c, &lv, &t->lv[l].plan);
+        // the level's plan is tuned only once its epilogue is the fused
+        // downward pass it runs with in the solve
+        octofmm_plan* lp = nullptr;
+        octofmm_status s = level_plan_build(c, &lv, &lp);
+        if (s == OCTOFMM_OK) {
+            lp->l2l = true;
+            lp->t_parent = t->lv[l].d_parent;
+            lp->t_octant = t->lv[l].d_octant;
+            lp->t_pL = l > 0 ? t->lv[l - 1].L : nullptr;
+            lp->t_pLc = l > 0 ? t->lv[l - 1].Lc : nullptr;
+            lp->t_pcom = l > 0 ? t->lv[l - 1].com : nullptr;
+            lp->t_g = t->lv[l].g;
+            lp->t_G = t->G;
+            lp->t_correct_am = t->correct_am;
+            s = level_plan_finish(lp, lv.flags);
+        }
         if (s != OCTOFMM_OK) {
             const std::string msg = c->err;
             tree_free(t);
             return fail(c, s, "level %d: %s", l, msg.c_str());
         }
-        octofmm_plan* lp = t->lv[l].plan;
-        lp->l2l = true;
-        lp->t_parent = t->lv[l].d_parent;
-        lp->t_octant = t->lv[l].d_octant;
-        lp->t_pL = l > 0 ? t->lv[l - 1].L : nullptr;
-        lp->t_pLc = l > 0 ? t->lv[l - 1].Lc : nullptr;
-        lp->t_pcom = l > 0 ? t->lv[l - 1].com : nullptr;
-        lp->t_g = t->lv[l].g;
-        lp->t_G = t->G;
-        lp->t_correct_am = t->correct_am;
+        t->lv[l].plan = lp;
         const octofmm_plan_stats& ps = t->lv[l].plan->stats;
         if (t->lv[l].plan->root)
             t->stats.root_pairs += ps.far_pairs;

@@ -73,23 +73,23 @@ struct Tables {
 
 // Level-kernel variants: lane mapping x 100 + waves per workgroup.
 //   mapping 0: a wave owns a 2x2x2 block of coarse cells, lane = (coarse cell,
-//              child); the 8 lanes of a coarse cell share every partner load.
-//   mapping 1: a wave owns one child class over all 64 coarse cells; its
-//              partner list is uniform, so no lane idles on near pairs.
-//   mapping 2: mapping 0 with each coarse offset's partner records staged
-//              once per wave in LDS and read back as broadcasts.
-//   mapping 3: mapping 2 with the staging done by direct global->LDS loads
-//              (no staging registers); 4 or 2 waves per workgroup so that 3
-//              waves per SIMD fit the 12 KiB LDS image of each wave.
-//   mapping 4: mapping 3 with the image double buffered by child halves.
+//              child); the 8 lanes of a coarse cell share every partner load
+//              (8, 4, 2 waves per workgroup).  The per-sub-grid drop-ins run
+//              variant 8.
+//   mapping 3: the partner records of each coarse offset staged once per wave
+//              in LDS by direct global->LDS loads and read back as 8-lane
+//              broadcasts; 4 or 2 waves per workgroup so that 3 waves per
+//              SIMD fit the 12 KiB LDS image of each wave.
 //   mapping 5: the coarse offsets of each 2x2x2 coarse block split over 4
-//              waves (variant 504), for levels with few sub-grids.  A 2-way
-//              split was measured slower at every level size tried.
+//              waves (variant 504), for levels with few sub-grids.
 //   mapping 7: mapping 3's staging with the coarse offsets of a block split
 //              over 4, 2 or 8 waves (variants 704, 702, 708); the partial sums
 //              meet in the waves' images.  The 8-wave split (one workgroup per
 //              CU by its LDS) is for levels with very few sub-grids.
-constexpr int kVariants[17] = {8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702, 708};
+// Mappings 1, 2 and 4 were measured and are not built (slower than mapping 3
+// or 7 at every level size: one child class per wave, register-staged LDS
+// images, double-buffered images).
+constexpr int kVariants[9] = {8, 4, 2, 304, 302, 504, 704, 702, 708};
 
 // Builds the tables; returns false when theta is below the floor or the
 // stencil fails the reference's symmetry / width checks.

@@ -133,7 +133,7 @@ def test_level_mixed_kinds(ctx, t, graph):
 
 
 @pytest.mark.parametrize("theta", [0.34, 0.45])
-@pytest.mark.parametrize("variant", [8, 108, 208, 304, 404, 504, 704, 708])
+@pytest.mark.parametrize("variant", [8, 304, 504, 704, 708])
 def test_level_mixed_kinds_other_theta(theta, variant, monkeypatch):
     """Wider stencils (theta 0.34: coarse offsets up to 2, p2p window 18^3)
     under every lane mapping, the staged ones included."""
@@ -175,7 +175,7 @@ def test_level_config2_full_size(ctx):
 
 
 @pytest.mark.parametrize("t", [_abi.RHO, _abi.DRHODT])
-@pytest.mark.parametrize("variant", [8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702, 708])
+@pytest.mark.parametrize("variant", [8, 4, 2, 304, 302, 504, 704, 702, 708])
 def test_level_kernel_variants(ctx, variant, t, monkeypatch):
     """Every lane mapping and workgroup shape of the level kernel gives the
     oracle's result (mixed level: M2L far + near passes, p2p, p2m), for both
@@ -194,7 +194,7 @@ def test_level_kernel_variants(ctx, variant, t, monkeypatch):
 
 
 @pytest.mark.parametrize("theta", [0.5, 0.34])
-@pytest.mark.parametrize("variant", [8, 108, 208, 304, 404, 504, 704, 708])
+@pytest.mark.parametrize("variant", [8, 304, 504, 704, 708])
 def test_level_edge_targets_missing_neighbours(theta, variant, monkeypatch):
     """Targets on the faces, edges and corners of the domain, whose partner
     coarse cells beyond it do not exist: every lane mapping must leave those
@@ -218,7 +218,7 @@ def test_autotuned_plan_leaves_outputs_untouched_until_executed(ctx):
     lv = workloads.uniform_level((4, 4, 4), ((1, 3),) * 3, seed=21)
     dl = hip.LevelOnDevice(lv)
     plan = dl.plan(ctx, type_=_abi.RHO, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
-    assert plan.stats()["variant"] in (8, 4, 2, 108, 104, 102, 208, 204, 202, 304, 302, 404, 402, 504, 704, 702, 708)
+    assert plan.stats()["variant"] in (8, 4, 2, 304, 302, 504, 704, 702, 708)
     assert not dl.L.numpy().any() and not dl.L_c.numpy().any()
     stream = hip.Stream()
     plan.execute(stream.handle)
@@ -334,3 +334,24 @@ def test_partial_halo_scattered_on_device(ctx, name, pattern):
     Lf, Lcf, _, _ = _gpu_case(ctx, kind, full, sgf)
     assert np.array_equal(L, Lf) and np.array_equal(Lc, Lcf)
     assert normwise(L, Lr) <= TOL and normwise(Lc, Lcr) <= TOL
+
+
+def test_tuning_cache_by_level_shape(ctx, tmp_path, monkeypatch):
+    """Autotuned plans of one level shape take one variant: the first plan
+    times the variants and records its choice (appended to the file
+    OCTOFMM_TUNE_CACHE names), later plans of the shape reuse it."""
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("OCTOFMM_TUNE_CACHE", str(path))
+    lv = workloads.uniform_level((5, 5, 3), ((1, 4), (1, 4), (1, 2)), seed=57, name="tune-cache")
+    dl = hip.LevelOnDevice(lv)
+    variants = []
+    for _ in range(3):
+        plan = dl.plan(ctx, type_=_abi.RHO, flags=_abi.PLAN_AUTOTUNE)
+        variants.append(plan.stats()["variant"])
+        plan.close()
+    dl.free()
+    lines = path.read_text().split("\n")
+    keys = [ln.split() for ln in lines if f"/m{len(lv.targets)}/" in ln]
+    print(f"tuning cache: {variants} {keys}", flush=True)
+    assert len(set(variants)) == 1
+    assert len(keys) == 1 and int(keys[0][1]) == variants[0]
