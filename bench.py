@@ -115,6 +115,8 @@ def errors_vs(lv, L, Lc, Lr, Lcr):
         "median_elementwise_rel_err": float(np.median(elem)),
         "force_rel_err": ft["force"],
         "torque_rel_err": ft["torque"],
+        "force_rel_err_rounding_floor": ft["force_floor"],
+        "torque_rel_err_rounding_floor": ft["torque_floor"],
         "force_rel_err_per_cell_sum": ft["force_per_cell_sum"],
         "torque_rel_err_per_cell_sum": ft["torque_per_cell_sum"],
     }

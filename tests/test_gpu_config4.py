@@ -4,7 +4,9 @@ oracle, in both precisions, each plan autotuned as the bench runs it.
 
 Gates:
   FP64   per component normwise <= 1e-12; total force and torque of all target
-         cells within 1e-12 of the oracle's totals (|F - F_ref| / |F_ref|).
+         cells within 1e-12 of the oracle's totals (|F - F_ref| / |F_ref|),
+         or within the rounding floor 4 eps sum_i |F_i| where that is larger
+         (tests/metrics.py).
   mixed  FP32 pair arithmetic for far offsets with |d|^2 > 4 (cell units),
          FP64 accumulation and FP64 angular-momentum correction: total force
          and torque within 1e-6 of the oracle's totals, per component
@@ -14,7 +16,7 @@ import numpy as np
 import pytest
 
 import oracle
-from metrics import force_torque_errors, normwise
+from metrics import describe, force_torque_gate, normwise
 from octotiger_amd import _abi, hip, native, workloads
 
 pytestmark = pytest.mark.gpu
@@ -63,15 +65,15 @@ def test_level_config4_full_size(ctx, c4, precision):
     tg = lv.targets
     en = normwise(L[tg], Lr[tg])
     ec = normwise(Lc[tg], Lcr[tg])
-    ft = force_torque_errors(L[tg], Lc[tg], Lr[tg], Lcr[tg], lv.M[tg, :, 0], lv.com[tg])
+    tol = TOL if precision == _abi.PRECISION_FP64 else MIXED_FORCE_TOL
+    ok, ft = force_torque_gate(L[tg], Lc[tg], Lr[tg], Lcr[tg], lv.M[tg, :, 0], lv.com[tg], tol)
     print(f"config4 precision {precision} variant {st['variant']}: normwise {en:.3e} L_c {ec:.3e} "
-          f"force {ft['force']:.3e} torque {ft['torque']:.3e} "
-          f"(per-cell sums: {ft['force_per_cell_sum']:.3e} {ft['torque_per_cell_sum']:.3e})", flush=True)
+          f"{describe(ft)}", flush=True)
     assert ec <= TOL
     if precision == _abi.PRECISION_FP64:
         assert en <= TOL
-        assert ft["force"] <= TOL and ft["torque"] <= TOL
+        assert ok
     else:
         assert st["mixed_far_pairs"] > 0.9 * st["far_pairs"]
         assert 0 < en <= MIXED_COMPONENT_TOL
-        assert ft["force"] <= MIXED_FORCE_TOL and ft["torque"] <= MIXED_FORCE_TOL
+        assert ok

@@ -419,3 +419,35 @@ def test_partial_halo_equals_full_halo_with_absent_cells_zeroed(name, pattern):
         if pattern != "all_shuffled":
             L0, _ = _run_case(kind, lv, workloads.subgrid_view(lv, 13)[0], _abi.RHO, use_ref)
             assert not np.array_equal(L, L0)      # the left-out cells mattered
+
+
+@pytest.mark.skipif(not oracle.ref_simd_available(), reason="vector build of the reference harness not built")
+def test_force_torque_gate_separates_reordering_from_mistakes():
+    """The total force / torque gate (relative to the reference totals, with
+    the rounding floor 4 eps sum_i |F_i|) accepts a second FP64 evaluation that
+    differs only in summation order and contraction (the -O3 vector build of
+    the reference templates against the scalar oracle) on a level whose
+    totals cancel 10^3-fold, and rejects planted mistakes of the size a
+    dropped or mis-signed term makes."""
+    from metrics import force_torque_gate
+    lv = workloads.uniform_level((4, 4, 4), ((1, 3),) * 3, seed=77)
+    Lr, Lcr = oracle.level_interactions(lv)
+    Ls, Lcs = oracle.level_interactions_simd(lv)
+    tg = lv.targets
+    args = (lv.M[tg, :, 0], lv.com[tg])
+    ok, e = force_torque_gate(Ls[tg], Lcs[tg], Lr[tg], Lcr[tg], *args, 1e-12)
+    assert ok and e["force_cancellation"] > 100
+    assert e["force"] > 0 or e["torque"] > 0        # the two evaluations do differ
+    # the angular-momentum correction left out
+    ok, _ = force_torque_gate(Ls[tg], 0 * Lcs[tg], Lr[tg], Lcr[tg], *args, 1e-12)
+    assert not ok
+    # one correction component with its sign flipped
+    bad = Lcs[tg].copy()
+    bad[..., 2] *= -1
+    ok, _ = force_torque_gate(Ls[tg], bad, Lr[tg], Lcr[tg], *args, 1e-12)
+    assert not ok
+    # one cell's gradient off by one part in 10^9
+    badL = Ls[tg].copy()
+    badL[0, 100, 1] *= 1 + 1e-9
+    ok, _ = force_torque_gate(badL, Lcs[tg], Lr[tg], Lcr[tg], *args, 1e-12)
+    assert not ok
