@@ -151,6 +151,21 @@ octofmm_status octofmm_compute_multipole_interactions(octofmm_context* ctx,
 octofmm_status octofmm_compute_monopole_interactions(octofmm_context* ctx,
     const octofmm_subgrid* subgrid, int32_t contains_multipole_neighbor, double* L, double* L_c);
 
+/* octofmm_compute_multipole_interactions without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (tight per-node loops, code that must not call hipMalloc).  Same
+ * arguments, kernels, results (bit for bit) and synchronous behaviour; its
+ * device workspace is a static array in the library's code object, 16 MiB per
+ * GPU, held from the first octofmm_create on that GPU until the process
+ * ends.  The maximum is one call's inputs: a sub-grid and its 26 neighbours,
+ * each of at most 512 cells (larger counts are rejected with
+ * OCTOFMM_ERR_INVALID_ARGUMENT).  Every call on one GPU uses that GPU's one
+ * workspace, which keeps the last call's contents (each call rewrites what it
+ * reads); calls on one GPU are serialised within the process, and callers
+ * must not expect them to overlap. */
+octofmm_status octofmm_compute_multipole_interactions_noalloc(octofmm_context* ctx,
+    const octofmm_subgrid* subgrid, double* L, double* L_c);
+
 /* ---------------------------------------------------------------------------
  * Level batch: every sub-grid of one octree level in one device pass.
  * This has no single counterpart in the reference; it batches what

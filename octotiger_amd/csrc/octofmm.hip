@@ -1285,6 +1285,36 @@ SubgridWorkspace carve_workspace(char* base) {
 
 }   // namespace
 
+// Workspace of octofmm_compute_multipole_interactions_noalloc: one per-sub-grid
+// workspace (kWsBytes, 16.0 MiB) in the code object's device memory, so that
+// entry point never allocates.  Each GPU the library runs on holds its own
+// copy until the process ends.  In host code `scratch` names a host-side
+// stand-in, not a device address: host code uses only the address
+// hipGetSymbolAddress returns for the current device (scratch_address).
+static __device__ __align__(256) char scratch[kWsBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_setup_mu;
+char* g_scratch_ptr[kScratchDevices] = {};
+// calls of the no-allocation entry point on one device share its scratch:
+// they are serialised per device within the process
+std::mutex g_scratch_call_mu[kScratchDevices];
+
+// The device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller has made it current); null on failure.
+char* scratch_address(int device) {
+    if (device < 0 || device >= kScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_setup_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = static_cast<char*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+}   // namespace
+
 struct octofmm_context {
     int device = 0;
     double theta = 0.5;
@@ -1297,6 +1327,7 @@ struct octofmm_context {
     // first call) and the host-side metadata they upload, built at creation
     SubgridWorkspace ws;
     char* d_ws = nullptr;
+    char* d_scratch = nullptr;     // this device's `scratch` (set at creation)
     int32_t* h_meta = nullptr;     // items[2] nbr[27*27] kind[27] foff[27*3]
     double* h_fx = nullptr;        // [27][3]
     int2* h_idx = nullptr;         // [26][512] (cell, position) of partial halos
@@ -1736,6 +1767,11 @@ octofmm_status octofmm_create(const octofmm_config* cfg, octofmm_context** out) 
         octofmm_destroy(c);
         return OCTOFMM_ERR_HIP;
     }
+    c->d_scratch = scratch_address(c->device);
+    if (!c->d_scratch) {
+        octofmm_destroy(c);
+        return OCTOFMM_ERR_HIP;
+    }
     std::vector<int32_t> pdir, ppart, psdir, pslab, pspart;
     std::vector<int4> ptgt;
     build_p2m_lists(c->tab, pdir, ptgt, ppart, psdir, pslab, pspart, c->p2m_slab_max, c->p2m_opp_disjoint);
@@ -1820,6 +1856,29 @@ octofmm_status octofmm_compute_multipole_interactions(octofmm_context* c, const 
     s = load_subgrid(c, c->ws, in, false, root, flags);
     if (s != OCTOFMM_OK) return s;
     return launch_subgrid(c, c->ws, in->dx, in->type, flags, root, L, in->type == OCTOFMM_RHO ? L_c : nullptr);
+}
+
+// The same call with the workspace in `scratch` instead of the context's
+// allocation: the same kernels and copies, no device allocation.  Every
+// region the kernels read is written by this call first (the target and
+// handed-over neighbour slots, partial halos zeroed before their scatter, the
+// metadata; empty directions are marked absent and never read), so what a
+// previous call left in the scratch does not enter the result.
+octofmm_status octofmm_compute_multipole_interactions_noalloc(octofmm_context* c, const octofmm_subgrid* in,
+    double* L, double* L_c) {
+    if (!c) return OCTOFMM_ERR_INVALID_ARGUMENT;
+    int flags = 0;
+    bool root = false;
+    octofmm_status s = check_multipole_call(c, in, L, L_c, flags, root);
+    if (s != OCTOFMM_OK) return s;
+    if (!c->d_scratch) return fail(c, OCTOFMM_ERR_HIP, "no static workspace on device %d", c->device);
+    HIPC(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // launch checks below report this call's launches only
+    std::lock_guard<std::mutex> lock(g_scratch_call_mu[c->device]);
+    const SubgridWorkspace w = carve_workspace(c->d_scratch);
+    s = load_subgrid(c, w, in, false, root, flags);
+    if (s != OCTOFMM_OK) return s;
+    return launch_subgrid(c, w, in->dx, in->type, flags, root, L, in->type == OCTOFMM_RHO ? L_c : nullptr);
 }
 
 octofmm_status octofmm_compute_monopole_interactions(octofmm_context* c, const octofmm_subgrid* in,

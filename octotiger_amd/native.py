@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "liboctofmm.so")
 EXPORTED = [
     "octofmm_abi_version", "octofmm_status_string", "octofmm_last_error", "octofmm_create",
     "octofmm_destroy", "octofmm_compute_multipole_interactions",
+    "octofmm_compute_multipole_interactions_noalloc",
     "octofmm_compute_monopole_interactions", "octofmm_level_plan_create",
     "octofmm_level_plan_execute", "octofmm_level_plan_stats", "octofmm_level_plan_time_stages",
     "octofmm_level_plan_destroy", "octofmm_tree_plan_create", "octofmm_tree_plan_execute",
@@ -48,6 +49,8 @@ def lib():
         L.octofmm_create.argtypes = [ctypes.POINTER(_abi.Config), ctypes.POINTER(vp)]
         L.octofmm_destroy.argtypes = [vp]
         L.octofmm_compute_multipole_interactions.argtypes = [
+            vp, ctypes.POINTER(_abi.Subgrid), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.octofmm_compute_multipole_interactions_noalloc.argtypes = [
             vp, ctypes.POINTER(_abi.Subgrid), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.octofmm_compute_monopole_interactions.argtypes = [
             vp, ctypes.POINTER(_abi.Subgrid), ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
@@ -107,6 +110,12 @@ class Context:
     def multipole_interactions(self, subgrid, L, L_c):
         """L [512,20] accumulated, L_c [512,3] overwritten (RHO)."""
         self._check(lib().octofmm_compute_multipole_interactions(
+            self._h, ctypes.byref(subgrid), _abi.dptr(L), _abi.dptr(L_c)))
+
+    def multipole_interactions_noalloc(self, subgrid, L, L_c):
+        """As multipole_interactions, with the library's static device
+        workspace (no device allocation)."""
+        self._check(lib().octofmm_compute_multipole_interactions_noalloc(
             self._h, ctypes.byref(subgrid), _abi.dptr(L), _abi.dptr(L_c)))
 
     def monopole_interactions(self, subgrid, contains_multipole_neighbor, L, L_c):
