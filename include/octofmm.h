@@ -180,6 +180,23 @@ typedef enum octofmm_slot_kind {
     OCTOFMM_SLOT_LEAF = 1      /* leaf: holds monopoles (mon); seen by neighbours at cell centres */
 } octofmm_slot_kind;
 
+/* A halo slot of a level handed over partially, as a remote neighbour hands
+ * it over: only `count` cells, in the order grid::get_ilist_n_bnd lists them
+ * (octotiger/multipole_interactions/legacy/multipole_interaction_interface.hpp:177,257,
+ *  octotiger/monopole_interactions/legacy/p2m_interaction_interface.hpp:142,239).
+ * At every execute the plan writes the slot's rows of the level's d_M and
+ * d_com (non-leaf slot) or d_mon (leaf slot): zeros, then each listed cell
+ * from the compact arrays, a cell listed twice taking its later entry, which
+ * is what update_input stages from such a list. */
+typedef struct octofmm_partial_slot {
+    int32_t slot;
+    int32_t count;             /* entries, 0..512 */
+    const int32_t* cells;      /* host [count] flat cell indices x*64 + y*8 + z */
+    const double* d_M;         /* device [count][20] (non-leaf slot) */
+    const double* d_com;       /* device [count][3]  (non-leaf slot) */
+    const double* d_mon;       /* device [count]     (leaf slot) */
+} octofmm_partial_slot;
+
 typedef struct octofmm_level {
     int32_t nslots;            /* sub-grid slots in the device arrays */
     const int32_t* neighbors;  /* host [nslots][27] slot of the neighbour in raw direction
@@ -200,6 +217,12 @@ typedef struct octofmm_level {
     const double* d_mon;       /* [nslots][512]     monopoles (zeros in non-leaf slots) */
     double* d_L;               /* [nslots][512][20] accumulated for target slots */
     double* d_L_c;             /* [nslots][512][3]  overwritten for target slots (RHO) */
+    /* halo slots handed over partially (none: npartial = 0); each names a
+       slot once and no target; their rows of d_M / d_com / d_mon are
+       written by the plan (octofmm_partial_slot) */
+    int32_t npartial;
+    int32_t reserved;
+    const octofmm_partial_slot* partial;   /* host [npartial] */
 } octofmm_level;
 
 #define OCTOFMM_PLAN_USE_GRAPH 1   /* capture the level pass in one hipGraph */
@@ -290,13 +313,19 @@ typedef struct octofmm_tree_level {
                                   g = -G (L_1..3 + L_c) (L_c only with correct_am); DRHODT: phi only */
 } octofmm_tree_level;
 
+/* Tree flag: leaf cells get only g (and L_c for RHO); their L rows are left
+ * as they are.  grid::compute_expansions uses a leaf's L only to form its
+ * potential and gravity (src/grid_fmm.cpp:1613-1629), so callers that read g
+ * alone skip most of the leaf rows' write traffic.  Needs d_g on every level
+ * with leaves.  No counterpart in the reference, which keeps leaf L. */
+#define OCTOFMM_TREE_LEAF_G_ONLY 8
 typedef struct octofmm_tree {
     int32_t nlevels;
     const octofmm_tree_level* levels;
     int32_t type;              /* octofmm_solve_type */
     int32_t precision;         /* octofmm_precision of the non-root levels */
     int32_t mixed_far_d2;
-    int32_t flags;             /* OCTOFMM_PLAN_USE_GRAPH, OCTOFMM_PLAN_AUTOTUNE */
+    int32_t flags;             /* OCTOFMM_PLAN_USE_GRAPH, OCTOFMM_PLAN_AUTOTUNE, OCTOFMM_TREE_LEAF_G_ONLY */
     double G;                  /* gravitational constant of the force assembly */
     int32_t correct_am;        /* add L_c to g (opts().correct_am_grav) */
     int32_t reserved;

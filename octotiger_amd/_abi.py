@@ -35,6 +35,7 @@ SLOT_LEAF = 1
 PLAN_USE_GRAPH = 1
 PLAN_AUTOTUNE = 2
 PLAN_ROOT = 4
+TREE_LEAF_G_ONLY = 8
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -75,6 +76,17 @@ class Config(ctypes.Structure):
 CHECK_FINITE = 1
 
 
+class PartialSlot(ctypes.Structure):
+    _fields_ = [
+        ("slot", ctypes.c_int32),
+        ("count", ctypes.c_int32),
+        ("cells", _ip),
+        ("d_M", ctypes.c_void_p),
+        ("d_com", ctypes.c_void_p),
+        ("d_mon", ctypes.c_void_p),
+    ]
+
+
 class Level(ctypes.Structure):
     _fields_ = [
         ("nslots", ctypes.c_int32),
@@ -93,6 +105,9 @@ class Level(ctypes.Structure):
         ("d_mon", ctypes.c_void_p),
         ("d_L", ctypes.c_void_p),
         ("d_L_c", ctypes.c_void_p),
+        ("npartial", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("partial", ctypes.POINTER(PartialSlot)),
     ]
 
 

@@ -459,6 +459,7 @@ struct LevelArgs {
     double G;
     int32_t l2l;
     int32_t correct_am;
+    int32_t leaf_rows;      // whole-tree solves: 0 = leaf cells get g (and L_c) only, no L rows
     int32_t nitems;
     int32_t item0;          // first item of this launch
     double inv_dx;

@@ -233,8 +233,10 @@ __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int ce
     }
     double Lf[20], Cf[3];
     tree_final_row<RHO>(a, slot, cell, L, ncomp, C, write_c, Lf, Cf);
+    if (!leaf_target || a.leaf_rows) {
 #pragma unroll
-    for (int j = 0; j < 10; ++j) Lo[j] = make_double2(Lf[2 * j], Lf[2 * j + 1]);
+        for (int j = 0; j < 10; ++j) Lo[j] = make_double2(Lf[2 * j], Lf[2 * j + 1]);
+    }
     if (RHO) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) Co[d] = Cf[d];
@@ -257,13 +259,14 @@ __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int ce
 // doubles) are written by consecutive lanes in consecutive 16-byte chunks.
 // Written straight from the class waves, each store instruction would touch
 // 64 rows two cells apart.  L goes in two halves of 256 cells (40 KiB of
-// LDS), then L_c and g together (28 KiB).  Every thread of the workgroup
-// calls it; buf is the kernel's dynamic LDS, no longer in use.
+// LDS), then L_c and g together (28 KiB); without a.leaf_rows only L_c and
+// g.  Every thread of the workgroup calls it; buf is the kernel's dynamic
+// LDS, no longer in use.
 __device__ __forceinline__ void leaf_rows_coalesced(const LevelArgs& a, int slot, int cell, const double* Lf,
     const double* Cf, const double* g4, double* __restrict__ buf) {
     const int t = threadIdx.x;
     double2* b2 = reinterpret_cast<double2*>(buf);
-    for (int half = 0; half < 2; ++half) {
+    for (int half = 0; half < (a.leaf_rows ? 2 : 0); ++half) {   // uniform
         __syncthreads();
         if ((cell >> 8) == half) {
             double2* row = b2 + (cell & 255) * 10;
@@ -1118,6 +1121,49 @@ __global__ __launch_bounds__(256) void scatter_partial(ScatterArgs a) {
     }
 }
 
+// Partial halo slots of a level plan (octofmm_partial_slot): the same scatter
+// as scatter_partial, one block per slot, from descriptors in device memory.
+struct PartialSlot {
+    const double* M;        // [count][20] handed-over multipoles (non-leaf slot)
+    const double* com;      // [count][3]
+    const double* mon;      // [count]     (leaf slot)
+    const int2* idx;        // [n] (cell, position in the handed-over arrays), each cell once
+    int32_t slot;
+    int32_t n;
+    int32_t multipole;      // 1: multipoles and centres of mass, 0: monopoles
+    int32_t pad;
+};
+
+__global__ __launch_bounds__(256) void scatter_slots(const PartialSlot* __restrict__ d, double* M, double* com,
+    double* mon) {
+    const PartialSlot e = d[blockIdx.x];
+    const int t = threadIdx.x;
+    double* Ms = M + (size_t)e.slot * kCells * 20;
+    double* Cs = com + (size_t)e.slot * kCells * 3;
+    double* ms = mon + (size_t)e.slot * kCells;
+    if (e.multipole) {
+        for (int i = t; i < kCells * 20; i += 256) Ms[i] = 0.0;
+        for (int i = t; i < kCells * 3; i += 256) Cs[i] = 0.0;
+    } else {
+        for (int i = t; i < kCells; i += 256) ms[i] = 0.0;
+    }
+    __syncthreads();
+    for (int i = t; i < e.n; i += 256) {
+        const int2 q = e.idx[i];
+        if (e.multipole) {
+            const double* src = e.M + (size_t)q.y * 20;
+#pragma unroll
+            for (int c = 0; c < 20; ++c) Ms[q.x * 20 + c] = src[c];
+            const double* sc = e.com + (size_t)q.y * 3;
+            Cs[q.x * 3] = sc[0];
+            Cs[q.x * 3 + 1] = sc[1];
+            Cs[q.x * 3 + 2] = sc[2];
+        } else {
+            ms[q.x] = e.mon[q.y];
+        }
+    }
+}
+
 // geometric centre xbase + i*dx, rounded as written (grid_fmm.cpp:1669-1671)
 __device__ __forceinline__ double cell_centre(double xb, int i, double dx) {
 #pragma clang fp contract(off)
@@ -1377,6 +1423,11 @@ struct octofmm_plan {
     double t_G = 1.0;
     int32_t t_correct_am = 1;
     double* d_partial = nullptr;   // root: per-chunk partial expansions
+    int32_t leaf_rows = 1;         // whole-tree plans: 0 = leaf cells get g (and L_c) only
+    // partial halo slots: rebuilt in the caller's slot rows before staging
+    int32_t npartial = 0;
+    PartialSlot* d_pslots = nullptr;
+    int2* d_pidx = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     // a level with both kinds of items runs its leaf kernel on this stream,
@@ -1697,6 +1748,7 @@ static octofmm_status launch_subgrid(octofmm_context* c, const SubgridWorkspace&
         la.G = 1.0;
         la.l2l = 0;
         la.correct_am = 0;
+        la.leaf_rows = 1;
         la.nitems = 1;
         la.item0 = 0;
         la.inv_dx = 1.0 / dx;
@@ -1925,6 +1977,8 @@ static void plan_free(octofmm_plan* p) {
     (void)hipFree(p->d_items);
     (void)hipFree(p->d_mixbits);
     (void)hipFree(p->d_partial);
+    (void)hipFree(p->d_pslots);
+    (void)hipFree(p->d_pidx);
     (void)hipFree(p->d_stage_list);
     delete p;
 }
@@ -1934,6 +1988,11 @@ static void plan_free(octofmm_plan* p) {
 // the monopole-target items.  Bits 1 and 2 write disjoint sub-grids.
 static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mask) {
     octofmm_context* c = p->ctx;
+    if ((stage_mask & 1) && p->npartial > 0) {
+        hipLaunchKernelGGL(scatter_slots, dim3(p->npartial), dim3(256), 0, st, p->d_pslots,
+            const_cast<double*>(p->d_M), const_cast<double*>(p->d_com), const_cast<double*>(p->d_mon));
+        HIPC(c, hipGetLastError());
+    }
     if (stage_mask & 1) {
         StageArgs sa;
         sa.M = p->d_M;
@@ -1999,6 +2058,7 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         la.G = p->t_G;
         la.l2l = p->l2l ? 1 : 0;
         la.correct_am = p->t_correct_am;
+        la.leaf_rows = p->leaf_rows;
         la.inv_dx = 1.0 / p->dx;
         la.neg_inv_dx = -1.0 / p->dx;
         const bool rho = p->type == OCTOFMM_RHO;
@@ -2262,6 +2322,52 @@ static octofmm_status level_plan_build(octofmm_context* c, const octofmm_level* 
             st.n_leaf_items++;
         }
     }
+    // partial halo slots: each cell once, from its last entry (later entries
+    // win, as in update_input's loop over the list)
+    if (lv->npartial < 0 || (lv->npartial > 0 && !lv->partial))
+        return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad partial halo list");
+    if (root && lv->npartial > 0) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "a root plan has no halo");
+    std::vector<PartialSlot> pslots;
+    std::vector<size_t> poff;   // first (cell, position) pair of each partial slot in pidx
+    std::vector<int2> pidx;
+    {
+        std::vector<char> listed(ns, 0);
+        for (int q = 0; q < lv->npartial; ++q) {
+            const octofmm_partial_slot& e = lv->partial[q];
+            if (e.slot < 0 || e.slot >= ns) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad partial slot %d", q);
+            if (seen[e.slot]) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "partial slot %d is a target", e.slot);
+            if (listed[e.slot]) return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "slot %d listed twice as partial", e.slot);
+            listed[e.slot] = 1;
+            if (e.count < 0 || e.count > kCells || (e.count > 0 && !e.cells))
+                return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad cell list of partial slot %d", e.slot);
+            const bool mp = lv->kind[e.slot] == OCTOFMM_SLOT_NONLEAF;
+            if (e.count > 0 && (mp ? (!e.d_M || !e.d_com) : !e.d_mon))
+                return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "null data of partial slot %d", e.slot);
+            int last[kCells];
+            for (int i = 0; i < kCells; ++i) last[i] = -1;
+            for (int i = 0; i < e.count; ++i) {
+                if (e.cells[i] < 0 || e.cells[i] >= kCells)
+                    return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad cell in partial slot %d", e.slot);
+                last[e.cells[i]] = i;
+            }
+            PartialSlot d;
+            d.M = e.d_M;
+            d.com = e.d_com;
+            d.mon = e.d_mon;
+            d.idx = nullptr;   // set once pidx is on the device
+            poff.push_back(pidx.size());
+            d.slot = e.slot;
+            d.multipole = mp ? 1 : 0;
+            d.pad = 0;
+            d.n = 0;
+            for (int cell = 0; cell < kCells; ++cell)
+                if (last[cell] >= 0) {
+                    pidx.push_back(make_int2(cell, last[cell]));
+                    ++d.n;
+                }
+            pslots.push_back(d);
+        }
+    }
     // pair counts: per cell per coarse offset
     for (int b = 0; b < 4; ++b)
         for (const int2& it : buckets[b]) {
@@ -2356,6 +2462,18 @@ static octofmm_status level_plan_build(octofmm_context* c, const octofmm_level* 
         return cleanup(OCTOFMM_ERR_HIP, "plan table upload failed");
     if (root && hipMalloc((void**)&p->d_partial, sizeof(double) * kRootChunks * kCells * kRecDoubles) != hipSuccess)
         return cleanup(OCTOFMM_ERR_ALLOCATION, "root partial buffer allocation failed");
+    if (!pslots.empty()) {
+        if (hipMalloc((void**)&p->d_pidx, sizeof(int2) * std::max<size_t>(1, pidx.size())) != hipSuccess ||
+            hipMalloc((void**)&p->d_pslots, sizeof(PartialSlot) * pslots.size()) != hipSuccess)
+            return cleanup(OCTOFMM_ERR_ALLOCATION, "partial halo table allocation failed");
+        for (size_t q = 0; q < pslots.size(); ++q) pslots[q].idx = p->d_pidx + poff[q];
+        if ((!pidx.empty() &&
+                hipMemcpy(p->d_pidx, pidx.data(), sizeof(int2) * pidx.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemcpy(p->d_pslots, pslots.data(), sizeof(PartialSlot) * pslots.size(), hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return cleanup(OCTOFMM_ERR_HIP, "partial halo table upload failed");
+        p->npartial = (int32_t)pslots.size();
+    }
     if ((int)stage_list.size() < ns) {
         if (hipMalloc((void**)&p->d_stage_list, sizeof(int32_t) * std::max<size_t>(1, stage_list.size())) != hipSuccess ||
             (!stage_list.empty() && hipMemcpy(p->d_stage_list, stage_list.data(), sizeof(int32_t) * stage_list.size(),
@@ -2691,6 +2809,9 @@ octofmm_status octofmm_tree_plan_create(octofmm_context* c, const octofmm_tree* 
                 return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "non-leaf node %d on the finest level", n);
             if (L.octant[n] < 0 || L.octant[n] > 7)
                 return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT, "bad octant of node %d on level %d", n, l);
+            if ((tr->flags & OCTOFMM_TREE_LEAF_G_ONLY) && L.kind[n] == OCTOFMM_SLOT_LEAF && !L.d_g)
+                return fail(c, OCTOFMM_ERR_INVALID_ARGUMENT,
+                    "OCTOFMM_TREE_LEAF_G_ONLY needs d_g on level %d, which holds leaves", l);
         }
         if (l > 0) {
             const octofmm_tree_level& U = tr->levels[l - 1];
@@ -2814,6 +2935,7 @@ octofmm_status octofmm_tree_plan_create(octofmm_context* c, const octofmm_tree* 
             lp->t_g = t->lv[l].g;
             lp->t_G = t->G;
             lp->t_correct_am = t->correct_am;
+            lp->leaf_rows = (tr->flags & OCTOFMM_TREE_LEAF_G_ONLY) ? 0 : 1;
             s = level_plan_finish(lp, lv.flags);
         }
         if (s != OCTOFMM_OK) {

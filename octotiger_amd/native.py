@@ -125,11 +125,16 @@ class Context:
 
     def level_plan(self, level, d_M, d_com, d_mon, d_L, d_L_c, type_=_abi.RHO,
                    precision=_abi.PRECISION_FP64, mixed_far_d2=4, flags=_abi.PLAN_USE_GRAPH,
-                   targets=None):
+                   targets=None, partial=None):
         """Plan over a workloads.Level whose state lives in device buffers
-        (addresses as ints, e.g. torch tensor data_ptr())."""
+        (addresses as ints, e.g. torch tensor data_ptr()).  partial: halo
+        slots handed over partially, a list of dicts with 'slot', 'cells'
+        (int32 cell indices in list order) and the device addresses 'M' and
+        'com' (non-leaf slot) or 'mon' (leaf slot) of the compact arrays; the
+        plan rewrites those slots' rows of d_M / d_com / d_mon at every
+        execute (octofmm_partial_slot)."""
         return LevelPlan(self, level, d_M, d_com, d_mon, d_L, d_L_c, type_, precision, mixed_far_d2,
-                         flags, targets)
+                         flags, targets, partial)
 
     def measure_fma_rate(self, precision=_abi.PRECISION_FP64):
         """Dense vector FMA rate of the device in TFLOP/s (FP64, or unpacked
@@ -206,7 +211,7 @@ class TreePlan:
 
 class LevelPlan:
     def __init__(self, ctx, level, d_M, d_com, d_mon, d_L, d_L_c, type_, precision, mixed_far_d2, flags,
-                 targets):
+                 targets, partial=None):
         self.ctx = ctx
         self._keep = [np.ascontiguousarray(level.neighbors, dtype=np.int32),
                       np.ascontiguousarray(level.kind, dtype=np.int32),
@@ -226,6 +231,15 @@ class LevelPlan:
         lv.mixed_far_d2 = mixed_far_d2
         lv.flags = flags
         lv.d_M, lv.d_com, lv.d_mon, lv.d_L, lv.d_L_c = d_M, d_com, d_mon, d_L, d_L_c
+        if partial:
+            ps = (_abi.PartialSlot * len(partial))()
+            for e, d in zip(ps, partial):
+                cells = np.ascontiguousarray(d["cells"], dtype=np.int32)
+                self._keep.append(cells)
+                e.slot, e.count, e.cells = int(d["slot"]), int(cells.size), _abi.iptr(cells)
+                e.d_M, e.d_com, e.d_mon = d.get("M"), d.get("com"), d.get("mon")
+            self._keep.append(ps)
+            lv.npartial, lv.partial = len(partial), ps
         self._h = ctypes.c_void_p()
         ctx._check(lib().octofmm_level_plan_create(ctx._h, ctypes.byref(lv), ctypes.byref(self._h)))
         ctx._plans.add(self)

@@ -1,13 +1,15 @@
 """Run the whole-tree gravity solve on one GPU and print its timings as JSON.
 
     python -m octotiger_amd.solve [--tree c5|small|sphere] [--steps N] [--precision fp64|mixed] [--no-graph]
+                                  [--leaf-g-only]
 
 Builds the tree on the host (octotiger_amd.workloads), uploads the leaf masses,
 creates the tree plan (one hipGraph per solve, autotuned level kernels; with
 --no-graph the kernels are launched on each execute instead) and runs it N
 times between HIP events; then times its two phases separately:
 the upward M2M pass, and the interactions of every level with the downward
-L2L pass and the leaf force assembly fused into their kernels.  For the
+L2L pass and the leaf force assembly.  --leaf-g-only writes only g (and L_c)
+for leaf cells (OCTOFMM_TREE_LEAF_G_ONLY).  For the
 solid-sphere tree it also prints the 'pot' norms Octo-Tiger's regression
 test checks (workloads.sphere_pot_norms).
 """
@@ -24,6 +26,7 @@ def main(argv=None):
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--leaf-g-only", action="store_true")
     args = ap.parse_args(argv)
 
     from . import _abi, hip, native, workloads
@@ -35,6 +38,8 @@ def main(argv=None):
     ctx = native.Context(theta=0.5, device=0)
     dev = hip.TreeOnDevice(tree)
     flags = (0 if args.no_graph else _abi.PLAN_USE_GRAPH) | (0 if args.no_autotune else _abi.PLAN_AUTOTUNE)
+    if args.leaf_g_only:
+        flags |= _abi.TREE_LEAF_G_ONLY
     prec = _abi.PRECISION_MIXED if args.precision == "mixed" else _abi.PRECISION_FP64
     plan = dev.plan(ctx, precision=prec, flags=flags)
     s = hip.Stream()
@@ -56,7 +61,8 @@ def main(argv=None):
         extra["pot_norms"] = f"pot {l1:e} {l2:e}"
     pairs = sum(st[k] for k in ("far_pairs", "near_pairs", "p2p_pairs", "p2m_pairs", "root_pairs"))
     print(json.dumps({"tree": tree.name, "nodes_per_level": tree.info["nodes_per_level"],
-                      "precision": args.precision, "graph": not args.no_graph, "ms_per_solve": ms,
+                      "precision": args.precision, "graph": not args.no_graph,
+                      "leaf_g_only": args.leaf_g_only, "ms_per_solve": ms,
                       "phases_ms": {"upward_m2m": phases[0], "interactions_l2l_forces": phases[1]},
                       "cell_interactions_per_s": pairs / (ms * 1e-3), "stats": st, **extra}), flush=True)
     plan.close()

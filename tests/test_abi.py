@@ -49,3 +49,32 @@ def test_struct_layouts_match_header_sizes():
     assert ctypes.sizeof(_abi.Neighbor) == 40
     assert ctypes.sizeof(_abi.Subgrid) == 3 * 8 + 26 * 40 + 8 + 24 + 8
     assert ctypes.sizeof(_abi.PlanStats) == 6 * 8 + 4 * 4
+
+
+def test_struct_layouts_match_the_header_compiled_by_gcc(tmp_path):
+    """Every ctypes mirror has the size and field offsets gcc gives the C
+    structs of include/octofmm.h."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("no gcc")
+    structs = {"octofmm_neighbor": _abi.Neighbor, "octofmm_subgrid": _abi.Subgrid, "octofmm_config": _abi.Config,
+               "octofmm_partial_slot": _abi.PartialSlot, "octofmm_level": _abi.Level,
+               "octofmm_plan_stats": _abi.PlanStats, "octofmm_tree_level": _abi.TreeLevel,
+               "octofmm_tree": _abi.Tree, "octofmm_tree_stats": _abi.TreeStats}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "octofmm.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (l.split() for l in out if l)}
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)

@@ -190,3 +190,40 @@ def test_solid_sphere_reproduces_reference_regression(ctx):
     l1, l2 = workloads.sphere_pot_norms(t, [r["g"][..., 0] for r in got])
     assert f"pot {l1:e} {l2:e}" == "pot 4.470774e-01 3.176879e+00"
     dev.free()
+
+
+@pytest.mark.parametrize("name", ["small", "c5"])
+def test_leaf_g_only_writes_g_and_leaves_leaf_rows_alone(ctx, name):
+    """OCTOFMM_TREE_LEAF_G_ONLY: g and L_c of every cell, and L of non-leaf
+    cells, are bit for bit those of the full solve; leaf cells' L rows keep
+    what they held before the solve."""
+    t = workloads.small_tree() if name == "small" else workloads.config5()
+    print(f"leaf g-only tree {t.name}", flush=True)
+    _, d1, full = run(ctx, t)
+    dev = hip.TreeOnDevice(t)
+    for lv, d in zip(t.levels, dev.levels):
+        d["L"].upload(np.full((lv.nnodes, 512, 20), 7.0))
+    _, d2, got = run(ctx, t, dev=dev, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE | _abi.TREE_LEAF_G_ONLY)
+    for lv, a, b in zip(t.levels, full, got):
+        leaf = lv.kind == _abi.SLOT_LEAF
+        assert np.array_equal(a["g"], b["g"]) and np.array_equal(a["L_c"], b["L_c"])
+        assert np.array_equal(a["L"][~leaf], b["L"][~leaf])
+        assert (b["L"][leaf] == 7.0).all()
+    d1.free()
+    d2.free()
+
+
+def test_leaf_g_only_needs_g():
+    """The flag without a g array on a level with leaves is refused at plan
+    creation."""
+    c = native.Context(theta=0.5, device=0)
+    t = workloads.small_tree()
+    dev = hip.TreeOnDevice(t)
+    addr = dev.addresses()
+    for a in addr:
+        a.pop("g")
+    with pytest.raises(native.OctofmmError) as e:
+        c.tree_plan(t, addr, flags=_abi.PLAN_USE_GRAPH | _abi.TREE_LEAF_G_ONLY)
+    assert e.value.status == _abi.ERR_INVALID_ARGUMENT
+    dev.free()
+    c.close()
