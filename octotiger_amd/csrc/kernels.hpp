@@ -248,27 +248,16 @@ __device__ __forceinline__ void m2l_far_rec(const Rec& r, double X0, double X1, 
     }
 }
 
-// Mixed precision far pair: the separation is formed in FP64 and rounded to
-// FP32; the derivative tensor and the 20-component contraction run in FP32 and
-// accumulate into the FP32 partial sums Lf of the current coarse offset (at
-// most 8 pairs, added to the FP64 expansion by the caller).  The
-// angular-momentum correction stays FP64: 1/r is refined from the FP32
-// estimate by one FP64 Newton step and the octupole difference is formed from
-// the FP64 record.
+// FP32 half of a mixed-precision far pair: the derivative tensor of -1/r at
+// the separation (x, y, z), already rounded to FP32, and the 20-component
+// contraction with the factor-scaled partner multipole m, accumulated into
+// the FP32 partial sums Lf.  Returns the FP32 estimate of 1/r, from which the
+// FP64 correction starts.
 template <bool RHO>
-__device__ __forceinline__ void m2l_far_mixed(const Rec& r, double X0, double X1, double X2, double inv_mc0,
-    const double* mc, float* Lf, double* C) {
-    const double xd = X0 - r.v[10].x, yd = X1 - r.v[10].y, zd = X2 - r.v[11].x;
-    const float x = (float)xd, y = (float)yd, z = (float)zd;
-    float m[20];
-#pragma unroll
-    for (int i = 0; i < 10; ++i) {
-        m[2 * i] = (float)r.v[i].x;
-        m[2 * i + 1] = (float)r.v[i].y;
-    }
+__device__ __forceinline__ float m2l_fp32_core(float x, float y, float z, const float* m, float* Lf) {
     const float x00 = x * x, x11 = y * y, x22 = z * z;
     float r2 = fmaf(z, z, fmaf(y, y, x00));
-    r2 = r2 < 1.0e-20f ? 1.0e-20f : r2;
+    r2 = fmaxf(r2, 1.0e-20f);   // one v_max_f32; r2 of finite separations is never NaN
     const float ri = __builtin_amdgcn_rsqf(r2);
     const float r2inv = ri * ri;
     const float d1 = ri * r2inv;
@@ -307,46 +296,111 @@ __device__ __forceinline__ void m2l_far_mixed(const Rec& r, double X0, double X1
         Lf[2] = fmaf(-m[2], D[5] + D[7] + D[8], Lf[2]);
         Lf[3] = fmaf(-m[3], D[6] + D[8] + D[9], Lf[3]);
     }
-    {
-        float l0 = Lf[0], l1 = Lf[1], l2 = Lf[2], l3 = Lf[3];
-        l0 = fmaf(m[4], D[4], l0);   l1 = fmaf(m[4], D[10], l1);  l2 = fmaf(m[4], D[11], l2);  l3 = fmaf(m[4], D[12], l3);
-        l0 = fmaf(m[5], D[5], l0);   l1 = fmaf(m[5], D[11], l1);  l2 = fmaf(m[5], D[13], l2);  l3 = fmaf(m[5], D[14], l3);
-        l0 = fmaf(m[6], D[6], l0);   l1 = fmaf(m[6], D[12], l1);  l2 = fmaf(m[6], D[14], l2);  l3 = fmaf(m[6], D[15], l3);
-        l0 = fmaf(m[7], D[7], l0);   l1 = fmaf(m[7], D[13], l1);  l2 = fmaf(m[7], D[16], l2);  l3 = fmaf(m[7], D[17], l3);
-        l0 = fmaf(m[8], D[8], l0);   l1 = fmaf(m[8], D[14], l1);  l2 = fmaf(m[8], D[17], l2);  l3 = fmaf(m[8], D[18], l3);
-        l0 = fmaf(m[9], D[9], l0);   l1 = fmaf(m[9], D[15], l1);  l2 = fmaf(m[9], D[18], l2);  l3 = fmaf(m[9], D[19], l3);
+    float l0 = Lf[0], l1 = Lf[1], l2 = Lf[2], l3 = Lf[3];
+    l0 = fmaf(m[4], D[4], l0);   l1 = fmaf(m[4], D[10], l1);  l2 = fmaf(m[4], D[11], l2);  l3 = fmaf(m[4], D[12], l3);
+    l0 = fmaf(m[5], D[5], l0);   l1 = fmaf(m[5], D[11], l1);  l2 = fmaf(m[5], D[13], l2);  l3 = fmaf(m[5], D[14], l3);
+    l0 = fmaf(m[6], D[6], l0);   l1 = fmaf(m[6], D[12], l1);  l2 = fmaf(m[6], D[14], l2);  l3 = fmaf(m[6], D[15], l3);
+    l0 = fmaf(m[7], D[7], l0);   l1 = fmaf(m[7], D[13], l1);  l2 = fmaf(m[7], D[16], l2);  l3 = fmaf(m[7], D[17], l3);
+    l0 = fmaf(m[8], D[8], l0);   l1 = fmaf(m[8], D[14], l1);  l2 = fmaf(m[8], D[17], l2);  l3 = fmaf(m[8], D[18], l3);
+    l0 = fmaf(m[9], D[9], l0);   l1 = fmaf(m[9], D[15], l1);  l2 = fmaf(m[9], D[18], l2);  l3 = fmaf(m[9], D[19], l3);
 #pragma unroll
-        for (int k = 10; k < 20; ++k) l0 = fmaf(-m[k], D[k], l0);
-        Lf[0] = l0; Lf[1] = l1; Lf[2] = l2; Lf[3] = l3;
-        static constexpr int kRowB[6][3] = {{10, 11, 12}, {11, 13, 14}, {12, 14, 15}, {13, 16, 17}, {14, 17, 18},
-            {15, 18, 19}};
+    for (int k = 10; k < 20; ++k) l0 = fmaf(-m[k], D[k], l0);
+    Lf[0] = l0; Lf[1] = l1; Lf[2] = l2; Lf[3] = l3;
+    static constexpr int kRowB[6][3] = {{10, 11, 12}, {11, 13, 14}, {12, 14, 15}, {13, 16, 17}, {14, 17, 18},
+        {15, 18, 19}};
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            float t = fmaf(m[0], D[4 + i], Lf[4 + i]);
-            t = fmaf(-m[1], D[kRowB[i][0]], t);
-            t = fmaf(-m[2], D[kRowB[i][1]], t);
-            Lf[4 + i] = fmaf(-m[3], D[kRowB[i][2]], t);
-        }
-#pragma unroll
-        for (int i = 10; i < 20; ++i) Lf[i] = fmaf(m[0], D[i], Lf[i]);
+    for (int i = 0; i < 6; ++i) {
+        float t = fmaf(m[0], D[4 + i], Lf[4 + i]);
+        t = fmaf(-m[1], D[kRowB[i][0]], t);
+        t = fmaf(-m[2], D[kRowB[i][1]], t);
+        Lf[4 + i] = fmaf(-m[3], D[kRowB[i][2]], t);
     }
-    if constexpr (RHO) {
-        double r2d = fma(zd, zd, fma(yd, yd, xd * xd));
-        r2d = r2d < 1.0e-20 ? 1.0e-20 : r2d;
-        double rid = (double)ri;
-        rid = fma(rid, fma(-(0.5 * r2d) * rid, rid, 0.5), rid);
-        const double r2i = rid * rid;
-        const double e1 = rid * r2i;
-        const double e2 = (-3.0 * r2i) * e1;
-        const double e3 = (-5.0 * r2i) * e2;
-        const double n0c = r.v[0].x * inv_mc0;
-        double a[10];
 #pragma unroll
-        for (int k = 0; k < 10; ++k) {
-            const double mk = (k & 1) ? r.v[5 + (k >> 1)].y : r.v[5 + (k >> 1)].x;
-            a[k] = fma(-mc[k], n0c, mk);
+    for (int i = 10; i < 20; ++i) Lf[i] = fmaf(m[0], D[i], Lf[i]);
+    return ri;
+}
+
+// FP64 half of a mixed-precision far pair (RHO): the angular-momentum
+// correction at the FP64 separation, with 1/r refined from the FP32 estimate
+// ri by one FP64 Newton step, and the octupole difference formed from the
+// partner's FP64 mass m0 and scaled octupole oct[0..9].
+__device__ __forceinline__ void mixed_correction(double xd, double yd, double zd, float ri, double m0,
+    const double* oct, double inv_mc0, const double* mc, double* C) {
+    double r2d = fma(zd, zd, fma(yd, yd, xd * xd));
+    r2d = fmax(r2d, 1.0e-20);
+    double rid = (double)ri;
+    rid = fma(rid, fma(-(0.5 * r2d) * rid, rid, 0.5), rid);
+    const double r2i = rid * rid;
+    const double e1 = rid * r2i;
+    const double e2 = (-3.0 * r2i) * e1;
+    const double e3 = (-5.0 * r2i) * e2;
+    const double n0c = m0 * inv_mc0;
+    double a[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) a[k] = fma(-mc[k], n0c, oct[k]);
+    am_correction(xd, yd, zd, e2, e3, a, C);
+}
+
+// Mixed precision far pair from an FP64 record (lane mapping 0): the
+// separation is formed in FP64 and rounded to FP32, the record's components
+// are rounded to FP32 for the derivative tensor and the 20-component
+// contraction, which accumulate into the FP32 partial sums Lf of the current
+// coarse offset (at most 8 pairs, added to the FP64 expansion by the caller).
+// The angular-momentum correction stays FP64 (mixed_correction).
+template <bool RHO>
+__device__ __forceinline__ void m2l_far_mixed(const Rec& r, double X0, double X1, double X2, double inv_mc0,
+    const double* mc, float* Lf, double* C) {
+    const double xd = X0 - r.v[10].x, yd = X1 - r.v[10].y, zd = X2 - r.v[11].x;
+    float m[20];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        m[2 * i] = (float)r.v[i].x;
+        m[2 * i + 1] = (float)r.v[i].y;
+    }
+    const float ri = m2l_fp32_core<RHO>((float)xd, (float)yd, (float)zd, m, Lf);
+    if constexpr (RHO) {
+        double oct[10];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) oct[k] = (k & 1) ? r.v[5 + (k >> 1)].y : r.v[5 + (k >> 1)].x;
+        mixed_correction(xd, yd, zd, ri, r.v[0].x, oct, inv_mc0, mc, C);
+    }
+}
+
+// Mixed-precision partner record (192 bytes, 12 chunks of 16), written by the
+// staging kernel beside the FP64 record for mixed-precision plans: chunks 0-4
+// the factor-scaled multipole m0..m19 rounded to FP32, chunk 5 (Y0, Y1),
+// chunk 6 (Y2, m0), chunks 7-11 the factor-scaled octupole m10..m19 in FP64.
+constexpr int kRecmY01 = 5;
+constexpr int kRecmY2M0 = 6;
+constexpr int kRecmOct = 7;
+
+// Mixed precision far pair from a mixed-precision record r (in LDS): no
+// conversion of the multipole, the separation formed in FP64 and rounded.
+template <bool RHO>
+__device__ __forceinline__ void m2l_far_m32(const double2* __restrict__ r, double X0, double X1, double X2,
+    double inv_mc0, const double* mc, float* Lf, double* C) {
+    const double2 y01 = r[kRecmY01], y2m = r[kRecmY2M0];
+    const double xd = X0 - y01.x, yd = X1 - y01.y, zd = X2 - y2m.x;
+    float m[20];
+    const float4* f4 = reinterpret_cast<const float4*>(r);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const float4 v = f4[i];
+        m[4 * i] = v.x;
+        m[4 * i + 1] = v.y;
+        m[4 * i + 2] = v.z;
+        m[4 * i + 3] = v.w;
+    }
+    const float ri = m2l_fp32_core<RHO>((float)xd, (float)yd, (float)zd, m, Lf);
+    if constexpr (RHO) {
+        double oct[10];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double2 v = r[kRecmOct + k];
+            oct[2 * k] = v.x;
+            oct[2 * k + 1] = v.y;
         }
-        am_correction(xd, yd, zd, e2, e3, a, C);
+        mixed_correction(xd, yd, zd, ri, y2m.y, oct, inv_mc0, mc, C);
     }
 }
 
@@ -417,6 +471,7 @@ struct StageArgs {
     const double* fxbase;   // [nslots][3] frame origin for leaf-cell centres
     const int32_t* foff;    // [nslots][3] frame offset of the slot's cell (0,0,0)
     double* rec;            // [nslots][512][24]
+    double* recm;           // mixed-precision records [nslots][512][24], or null
     const int32_t* list;    // slots to stage (null: all nslots)
     double dx;
     int32_t nslots;         // slots to stage (length of list when given)
@@ -432,6 +487,11 @@ struct LevelArgs {
     const int2* items;      // (slot, flags)
     const Tables* tab;
     const uint64_t* mixbits;   // per coarse offset: byte k, bit j = far pair evaluated in FP32
+    // mixed-precision plans, lane mappings 3 and 7: the mixed-precision
+    // records, and per far coarse offset (far_delta order) 1 where every far
+    // pair of the offset is evaluated in FP32
+    const double* recm;
+    const uint8_t* mixoff;
     const double* mon;      // [nslots][512] monopoles (p2p window of leaf targets)
     // p2m pair lists by neighbour direction (raw 0..26): dir[2r] targets in
     // direction r starting at dir[2r+1] of tgt; tgt = (target cell, first

@@ -112,6 +112,42 @@ __global__ __launch_bounds__(256) void stage_records(StageArgs a) {
             }
         }
         dst[ch] = v;
+        if (a.recm) {
+            // the mixed-precision record of the same cell: m0..m19 rounded to
+            // FP32 (4 per chunk), (Y0, Y1), (Y2, m0), the octupole in FP64
+            double2 w = make_double2(0.0, 0.0);
+            if (k == OCTOFMM_SLOT_NONLEAF) {
+                if (part < 5) {
+                    float4 f;
+                    f.x = (float)(sM[cl * 20 + 4 * part] * record_scale(4 * part));
+                    f.y = (float)(sM[cl * 20 + 4 * part + 1] * record_scale(4 * part + 1));
+                    f.z = (float)(sM[cl * 20 + 4 * part + 2] * record_scale(4 * part + 2));
+                    f.w = (float)(sM[cl * 20 + 4 * part + 3] * record_scale(4 * part + 3));
+                    w = __builtin_bit_cast(double2, f);
+                } else if (part == kRecmY01) {
+                    w = make_double2(sC[cl * 3], sC[cl * 3 + 1]);
+                } else if (part == kRecmY2M0) {
+                    w = make_double2(sC[cl * 3 + 2], sM[cl * 20]);
+                } else {
+                    const int c = 10 + 2 * (part - kRecmOct);
+                    w = make_double2(sM[cl * 20 + c] * record_scale(c), sM[cl * 20 + c + 1] * record_scale(c + 1));
+                }
+            } else if (k == OCTOFMM_SLOT_LEAF) {
+                const double m = sm[cl];
+                const int* fo = a.foff + 3 * slot;
+                const double* fb = a.fxbase + 3 * slot;
+                if (part == 0) {
+                    w = __builtin_bit_cast(double2, make_float4((float)m, 0.0f, 0.0f, 0.0f));
+                } else if (part == kRecmY01) {
+                    w.x = (double)(x + fo[0]) * a.dx + fb[0] - kInx * a.dx;
+                    w.y = (double)(y + fo[1]) * a.dx + fb[1] - kInx * a.dx;
+                } else if (part == kRecmY2M0) {
+                    w.x = (double)(z + fo[2]) * a.dx + fb[2] - kInx * a.dx;
+                    w.y = m;
+                }
+            }
+            reinterpret_cast<double2*>(a.recm + row0 * kRecDoubles)[ch] = w;
+        }
     }
 }
 
@@ -512,7 +548,14 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(c
 // the other waves then hand their partial sums to wave 0 through their own
 // images, and wave 0 adds them in wave order, so the sum is the same on every
 // run.
-template <bool RHO, int SPLIT = 1>
+//
+// MIXED (mixed-precision plans): a far offset all of whose far pairs are
+// evaluated in FP32 (a.mixoff) stages the mixed-precision records instead and
+// runs m2l_far_m32 on them, with the offset's FP32 partial sums added to the
+// FP64 expansion after its 8 children; the other far offsets run in FP64.
+// The precision is uniform over the wave for each offset, so no lane waits
+// for the other precision's arithmetic.
+template <bool RHO, int SPLIT = 1, bool MIXED = false>
 __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const int* s_nbr, const int* s_kind,
     int w, double2* __restrict__ s_stage, int sl = 0) {
     // this wave's image (wave-uniform address)
@@ -534,14 +577,14 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     const auto* nbr_row = as_const(a.nbr + (size_t)__builtin_amdgcn_readfirstlane(slot) * 27);
     // partner coarse cell of group gi (coarse cell gi of the wave's block) at
     // offset a; empty directions are marked in bit gi of `empty`
-    auto group_base = [&](int gi, int ax, int ay, int az, unsigned& empty) -> const double2* {
+    auto group_base = [&](const double* src, int gi, int ax, int ay, int az, unsigned& empty) -> const double2* {
         const int px = bx + ((gi >> 2) & 1) + ax, py = by + ((gi >> 1) & 1) + ay, pz = bz + (gi & 1) + az;
         const int raw = (px < 0 ? 0 : (px < 4 ? 1 : 2)) + 3 * (py < 0 ? 0 : (py < 4 ? 1 : 2)) +
             9 * (pz < 0 ? 0 : (pz < 4 ? 1 : 2));
         const int ns = nbr_row[raw];
         const int cb = ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
         if (ns < 0) empty |= 1u << gi;
-        return reinterpret_cast<const double2*>(a.rec + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles);
+        return reinterpret_cast<const double2*>(src + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles);
     };
     // Group c (8 records = 96 chunks) is loaded by one full instruction
     // (chunks 0..63) and one on the lower 32 lanes (chunks 64..95): both are a
@@ -553,12 +596,12 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     // current image is still arriving, instead of one wait per group).
     const char* nbase[8];
     unsigned empty_next = 0;   // groups of the offset bases() last formed whose partner slot is empty
-    auto bases = [&](int di) {
+    auto bases = [&](int di, const double* src) {
         const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
         unsigned empty = 0;
 #pragma unroll
         for (int gi = 0; gi < 8; ++gi)
-            nbase[gi] = reinterpret_cast<const char*>(group_base(gi, ax, ay, az, empty));
+            nbase[gi] = reinterpret_cast<const char*>(group_base(src, gi, ax, ay, az, empty));
         asm volatile("" ::"s"(nbase[0]), "s"(nbase[1]), "s"(nbase[2]), "s"(nbase[3]), "s"(nbase[4]),
                      "s"(nbase[5]), "s"(nbase[6]), "s"(nbase[7]));
         empty_next = empty;
@@ -588,22 +631,38 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     // only the offsets with a far pair are staged (delta 0 is all near / self)
     const int nfd = tab->nfar_delta;
     const int q0 = SPLIT == 1 ? 0 : __builtin_amdgcn_readfirstlane(sl);
+    const auto* mixoff = as_const(a.mixoff);
+    // the records offset q is staged from
+    auto source = [&](int q) -> const double* { return (MIXED && mixoff[q]) ? a.recm : a.rec; };
     if (q0 < nfd) {
-        bases(tab->far_delta[q0]);
+        bases(tab->far_delta[q0], source(q0));
         issue();
     }
     for (int q = q0; q < nfd; q += SPLIT) {
         const unsigned fb = far_bits(tab->far_delta[q], empty_next);
-        if (q + SPLIT < nfd) bases(tab->far_delta[q + SPLIT]);
+        const bool f32 = MIXED && mixoff[q];   // uniform
+        if (q + SPLIT < nfd) bases(tab->far_delta[q + SPLIT], source(q + SPLIT));
         __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): this wave's image is in LDS
         __builtin_amdgcn_wave_barrier();
+        if (MIXED && f32) {
+            float Lf[20];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            if (fb & (1u << j)) {
-                Rec cur;
+            for (int i = 0; i < 20; ++i) Lf[i] = 0.0f;
 #pragma unroll
-                for (int i = 0; i < 12; ++i) cur.v[i] = group[j * 12 + i];
-                m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
+            for (int j = 0; j < 8; ++j)
+                if (fb & (1u << j)) m2l_far_m32<RHO>(group + j * 12, X0, X1, X2, inv_mc0, mc, Lf, C);
+            // the offset's FP32 partial sums join the FP64 expansion
+#pragma unroll
+            for (int i = 0; i < 20; ++i) L[i] += (double)Lf[i];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (fb & (1u << j)) {
+                    Rec cur;
+#pragma unroll
+                    for (int i = 0; i < 12; ++i) cur.v[i] = group[j * 12 + i];
+                    m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
+                }
             }
         }
         lds_wave_sync();   // every read of the image done before it is refilled
@@ -935,7 +994,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
 
 // Mapping-7 kernel: one workgroup of SPLIT waves per 2x2x2 coarse block, the
 // coarse offsets split over the waves (mapping 3's staging in each).
-template <bool RHO, int SPLIT>
+template <bool RHO, int SPLIT, bool MIXED = false>
 __global__ __launch_bounds__(64 * SPLIT) __attribute__((amdgpu_waves_per_eu(3))) void level_kernel_dsplit(LevelArgs a) {
     __shared__ int s_nbr[27];
     __shared__ int s_kind[27];
@@ -950,7 +1009,7 @@ __global__ __launch_bounds__(64 * SPLIT) __attribute__((amdgpu_waves_per_eu(3)))
     }
     __syncthreads();
     if (it.y & kItemLeaf) return;   // uniform per workgroup
-    m2l_item_direct<RHO, SPLIT>(a, it.x, it.y, s_nbr, s_kind, w, s_stage, (int)(threadIdx.x >> 6));
+    m2l_item_direct<RHO, SPLIT, MIXED>(a, it.x, it.y, s_nbr, s_kind, w, s_stage, (int)(threadIdx.x >> 6));
 }
 
 template <bool RHO>
@@ -962,6 +1021,15 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
         switch (variant) {
         case 8: hipLaunchKernelGGL((level_kernel<RHO, 8, true>), grid, block, 0, st, la); break;
         case 4: hipLaunchKernelGGL((level_kernel<RHO, 4, true>), grid, block, 0, st, la); break;
+        case 704:
+            hipLaunchKernelGGL((level_kernel_dsplit<RHO, 4, true>), dim3(nitems * 8), dim3(256), 0, st, la);
+            break;
+        case 702:
+            hipLaunchKernelGGL((level_kernel_dsplit<RHO, 2, true>), dim3(nitems * 8), dim3(128), 0, st, la);
+            break;
+        case 708:
+            hipLaunchKernelGGL((level_kernel_dsplit<RHO, 8, true>), dim3(nitems * 8), dim3(512), 0, st, la);
+            break;
         default: hipLaunchKernelGGL((level_kernel<RHO, 2, true>), grid, block, 0, st, la); break;
         }
         return;
@@ -1411,6 +1479,12 @@ struct octofmm_plan {
     int32_t mixed_far_d2 = 4;
     std::vector<int2> h_items;     // the work items (host copy, for the tuning key)
     uint64_t* d_mixbits = nullptr;
+    // mixed precision, lane mappings 3 and 7: the mixed-precision records and
+    // the far offsets evaluated in FP32 as a whole (far_delta order)
+    double* d_recm = nullptr;
+    uint8_t* d_mixoff = nullptr;
+    int64_t mixed_far_pairs_by_offset = 0;   // FP32 far pairs under that rule
+    int64_t mixed_far_pairs_by_pair = 0;     // and under the per-pair rule of mapping 0
     bool root = false;             // root sub-grid: root stencil over slot 0
     // set by a whole-tree plan: write final expansions (fused downward pass)
     bool l2l = false;
@@ -1693,6 +1767,7 @@ static octofmm_status launch_subgrid(octofmm_context* c, const SubgridWorkspace&
     sa.fxbase = w.fx;
     sa.foff = w.meta + kMetaFoff;
     sa.rec = w.rec;
+    sa.recm = nullptr;
     sa.list = nullptr;
     sa.dx = dx;
     sa.nslots = 27;
@@ -1734,6 +1809,8 @@ static octofmm_status launch_subgrid(octofmm_context* c, const SubgridWorkspace&
         la.items = reinterpret_cast<const int2*>(w.meta + kMetaItems);
         la.tab = c->d_tab;
         la.mixbits = nullptr;
+        la.recm = nullptr;
+        la.mixoff = nullptr;
         la.mon = w.mon;
         la.p2m_dir = c->d_p2m_dir;
         la.p2m_tgt = c->d_p2m_tgt;
@@ -1976,6 +2053,8 @@ static void plan_free(octofmm_plan* p) {
     (void)hipFree(p->d_fxbase);
     (void)hipFree(p->d_items);
     (void)hipFree(p->d_mixbits);
+    (void)hipFree(p->d_recm);
+    (void)hipFree(p->d_mixoff);
     (void)hipFree(p->d_partial);
     (void)hipFree(p->d_pslots);
     (void)hipFree(p->d_pidx);
@@ -2002,6 +2081,7 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         sa.fxbase = p->d_fxbase;
         sa.foff = p->d_foff;
         sa.rec = p->d_rec;
+        sa.recm = p->d_recm;
         sa.list = p->d_stage_list;
         sa.dx = p->dx;
         sa.nslots = p->d_stage_list ? p->n_stage : p->nslots;
@@ -2042,6 +2122,8 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         la.items = p->d_items;
         la.tab = c->d_tab;
         la.mixbits = p->d_mixbits;
+        la.recm = p->d_recm;
+        la.mixoff = p->d_mixoff;
         la.mon = p->d_mon;
         la.p2m_dir = c->d_p2m_dir;
         la.p2m_tgt = c->d_p2m_tgt;
@@ -2121,11 +2203,18 @@ static octofmm_status plan_run(octofmm_plan* p, hipStream_t st, int mask) {
 }   // extern "C"
 
 // Kernel variants a plan may run: FP64 plans every variant, mixed-precision
-// plans the mapping-0 forms (8, 4, 2 waves per workgroup).
+// plans the mapping-0 forms (8, 4, 2 waves per workgroup; FP32 chosen per
+// pair) and mapping 7 (704, 702, 708; FP32 chosen per coarse offset).
 static bool variant_allowed(const octofmm_plan* p, int v) {
     bool known = false;
     for (int k : kVariants) known = known || k == v;
-    return known && !(p->precision == OCTOFMM_PRECISION_MIXED && v >= 100);
+    return known && !(p->precision == OCTOFMM_PRECISION_MIXED && v >= 100 && v / 100 != 7);
+}
+
+// The plan's count of FP32 far pairs follows the chosen variant's rule.
+static void set_mixed_count(octofmm_plan* p) {
+    if (p->precision == OCTOFMM_PRECISION_MIXED)
+        p->stats.mixed_far_pairs = p->variant / 100 == 7 ? p->mixed_far_pairs_by_offset : p->mixed_far_pairs_by_pair;
 }
 
 // Tuning cache: the variant the autotuner chose, keyed by level shape (device
@@ -2496,11 +2585,23 @@ static octofmm_status level_plan_build(octofmm_context* c, const octofmm_level* 
         if (hipMalloc((void**)&p->d_mixbits, sizeof(uint64_t) * mb.size()) != hipSuccess ||
             hipMemcpy(p->d_mixbits, mb.data(), sizeof(uint64_t) * mb.size(), hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(OCTOFMM_ERR_ALLOCATION, "mixed-precision table upload failed");
-        // FP32 pair count (per target cell class, summed over the M2L items)
-        int64_t per_cell[8] = {0};
-        for (int k = 0; k < 8; ++k)
-            for (int di = 0; di < T.ndelta; ++di) per_cell[k] += __builtin_popcountll((mb[di] >> (8 * k)) & 0xffull);
-        p->stats.mixed_far_pairs = 0;
+        // lane mappings 3 and 7 choose the precision per far coarse offset:
+        // FP32 where every far pair of the offset is an FP32 pair above
+        std::vector<uint8_t> mo(T.nfar_delta > 0 ? T.nfar_delta : 1, 0);
+        std::vector<char> whole(T.ndelta > 0 ? T.ndelta : 1, 0);
+        for (int q = 0; q < T.nfar_delta; ++q) {
+            const int di = T.far_delta[q];
+            whole[di] = T.farbits[di] != 0 && mb[di] == T.farbits[di];
+            mo[q] = whole[di] ? 1 : 0;
+        }
+        if (hipMalloc((void**)&p->d_mixoff, mo.size()) != hipSuccess ||
+            hipMemcpy(p->d_mixoff, mo.data(), mo.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMalloc((void**)&p->d_recm, (size_t)ns * kCells * kRecDoubles * sizeof(double) + kRecTailBytes) !=
+                hipSuccess)
+            return cleanup(OCTOFMM_ERR_ALLOCATION, "mixed-precision record allocation failed");
+        // FP32 pair counts under both rules, summed over the M2L items
+        p->mixed_far_pairs_by_pair = 0;
+        p->mixed_far_pairs_by_offset = 0;
         for (const int2& it : items) {
             if (it.y & kItemLeaf) continue;
             for (int cx = 0; cx < 4; ++cx)
@@ -2511,10 +2612,11 @@ static octofmm_status level_plan_build(octofmm_context* c, const octofmm_level* 
                             const int raw = (px < 0 ? 0 : px < 4 ? 1 : 2) + 3 * (py < 0 ? 0 : py < 4 ? 1 : 2) +
                                 9 * (pz < 0 ? 0 : pz < 4 ? 1 : 2);
                             if (lv->neighbors[(size_t)it.x * 27 + raw] < 0) continue;
-                            p->stats.mixed_far_pairs += __builtin_popcountll(mb[di]);
+                            p->mixed_far_pairs_by_pair += __builtin_popcountll(mb[di]);
+                            if (whole[di]) p->mixed_far_pairs_by_offset += __builtin_popcountll(T.farbits[di]);
                         }
         }
-        (void)per_cell;
+        set_mixed_count(p);
     }
     p->mixed_far_d2 = lv->mixed_far_d2;
     p->h_items = items;
@@ -2549,6 +2651,7 @@ static octofmm_status level_plan_finish(octofmm_plan* p, int flags) {
         }
     }
     p->stats.variant = p->variant;
+    set_mixed_count(p);
     if (flags & OCTOFMM_PLAN_USE_GRAPH) {
         // the side stream must exist before the capture starts
         octofmm_status fs = plan_fork_resources(p);

@@ -7,8 +7,9 @@ Gates:
          cells within 1e-12 of the oracle's totals (|F - F_ref| / |F_ref|),
          or within the rounding floor 4 eps sum_i |F_i| where that is larger
          (tests/metrics.py).
-  mixed  FP32 pair arithmetic for far offsets with |d|^2 > 4 (cell units),
-         FP64 accumulation and FP64 angular-momentum correction: total force
+  mixed  FP32 pair arithmetic for far pairs with |d|^2 > 4 (cell units; the
+         mapping-7 kernels take whole coarse offsets whose far pairs all
+         qualify), FP64 accumulation and FP64 angular-momentum correction: total force
          and torque within 1e-6 of the oracle's totals, per component
          normwise <= 1e-5, L_c normwise <= 1e-12.
 """
@@ -74,6 +75,8 @@ def test_level_config4_full_size(ctx, c4, precision):
         assert en <= TOL
         assert ok
     else:
-        assert st["mixed_far_pairs"] > 0.9 * st["far_pairs"]
+        # per pair (mapping 0) 96.8% of the far pairs are FP32, per coarse
+        # offset (mapping 7: the face offsets stay FP64) 81%
+        assert st["mixed_far_pairs"] > (0.8 if st["variant"] // 100 == 7 else 0.96) * st["far_pairs"]
         assert 0 < en <= MIXED_COMPONENT_TOL
         assert ok
