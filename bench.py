@@ -61,8 +61,10 @@ def kernel_name(variant, rho=True):
         return f"level_kernel_split<{t}, {waves}>"
     if mapping == 7:
         return f"level_kernel_dsplit<{t}, {waves}>"
-    if mapping >= 3:
-        return f"level_kernel_direct<{t}, {waves}, {mapping}>"
+    if mapping == 8:
+        return f"level_kernel_dqueue<{t}, {waves}>"
+    if mapping == 3:
+        return f"level_kernel_direct<{t}, {waves}>"
     return f"level_kernel<{t}, {waves}, {mapping}>"
 
 

@@ -229,7 +229,9 @@ typedef struct octofmm_level {
 #define OCTOFMM_PLAN_AUTOTUNE 2    /* time the kernel variants once and keep the fastest */
 /* The environment variable OCTOFMM_VARIANT, when it names a variant of the
  * multipole-target kernel (lane mapping x 100 + waves per workgroup: 8, 4, 2,
- * 304, 302, 504, 704, 702, 708; mixed precision: 8, 4, 2), fixes that variant
+ * 304, 302, 504, 704, 702, 708, and 804, 802 for mapping 7 as one persistent
+ * grid over per-XCD work counters; mixed precision: 8, 4, 2, 704, 702, 708,
+ * 804, 802), fixes that variant
  * for every plan created while it is set, in place of the default or the
  * autotuner's choice.  All variants give the same results to rounding; it
  * exists for measurement.

@@ -520,6 +520,9 @@ struct LevelArgs {
     int32_t l2l;
     int32_t correct_am;
     int32_t leaf_rows;      // whole-tree solves: 0 = leaf cells get g (and L_c) only, no L rows
+    // persistent form of lane mapping 7: 8 work counters, one per XCD
+    // (cleared by queue_reset before each launch)
+    uint32_t* queue;
     int32_t nitems;
     int32_t item0;          // first item of this launch
     double inv_dx;

@@ -1012,6 +1012,66 @@ __global__ __launch_bounds__(64 * SPLIT) __attribute__((amdgpu_waves_per_eu(3)))
     m2l_item_direct<RHO, SPLIT, MIXED>(a, it.x, it.y, s_nbr, s_kind, w, s_stage, (int)(threadIdx.x >> 6));
 }
 
+// Persistent form of the mapping-7 kernel (variants 804, 802): a grid of
+// the workgroups the chip holds at once, each taking work units (sub-grid,
+// coarse block) from a counter until none are left.  Workgroups are handed
+// to the 8 XCDs round-robin, so workgroup b runs on XCD b % 8 and takes its
+// units from that XCD's counter, over the XCD's contiguous share of the units
+// (the same split xcd_item makes): neighbouring sub-grids still share an L2.
+// Every workgroup leaves the loop once its XCD's counter has passed the end
+// of the share, so the grid drains.
+__global__ void queue_reset(uint32_t* q) {
+    if (threadIdx.x < 8) q[threadIdx.x] = 0u;
+}
+
+template <bool RHO, int SPLIT, bool MIXED = false>
+__global__ __launch_bounds__(64 * SPLIT) __attribute__((amdgpu_waves_per_eu(3))) void level_kernel_dqueue(LevelArgs a) {
+    __shared__ int s_nbr[27];
+    __shared__ int s_kind[27];
+    __shared__ int s_unit;
+    __shared__ double2 s_stage[SPLIT * kDirWave];
+    const int n = 8 * a.nitems;                 // units: 8 coarse blocks per item
+    const int x = blockIdx.x & 7;
+    const int q = n >> 3, r = n & 7;
+    const int lo = x * q + (x < r ? x : r), hi = lo + q + (x < r ? 1 : 0);
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = lo + (int)atomicAdd(a.queue + x, 1u);
+        __syncthreads();
+        const int v = s_unit;
+        if (v >= hi) break;                     // uniform
+        const int2 it = a.items[a.item0 + (v >> 3)];
+        if (threadIdx.x < 27) {
+            const int nb = a.nbr[(size_t)it.x * 27 + threadIdx.x];
+            s_nbr[threadIdx.x] = nb;
+            s_kind[threadIdx.x] = nb >= 0 ? a.kind[nb] : -1;
+        }
+        __syncthreads();
+        if (!(it.y & kItemLeaf))   // uniform per workgroup
+            m2l_item_direct<RHO, SPLIT, MIXED>(a, it.x, it.y, s_nbr, s_kind, v & 7, s_stage, (int)(threadIdx.x >> 6));
+        __syncthreads();   // the unit's LDS (images, tables, s_unit) is free again
+    }
+}
+
+template <bool RHO, int SPLIT, bool MIXED>
+void launch_dqueue(int nitems, hipStream_t st, const LevelArgs& la) {
+    // workgroups the device holds at once (occupancy of this instantiation
+    // times CUs), found once per process
+    static const int capacity = [] {
+        int dev = 0, per = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, level_kernel_dqueue<RHO, SPLIT, MIXED>, 64 * SPLIT, 0) ==
+                hipSuccess && per > 0)
+            return per * prop.multiProcessorCount;
+        return 256;
+    }();
+    const int units = 8 * nitems;
+    int grid = std::min(units, capacity);
+    grid = std::max(8, (grid + 7) & ~7);   // every XCD's counter has workgroups
+    hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, st, la.queue);
+    hipLaunchKernelGGL((level_kernel_dqueue<RHO, SPLIT, MIXED>), dim3(grid), dim3(64 * SPLIT), 0, st, la);
+}
+
 template <bool RHO>
 void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, bool mixed = false) {
     const int waves = variant % 100;
@@ -1030,6 +1090,8 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
         case 708:
             hipLaunchKernelGGL((level_kernel_dsplit<RHO, 8, true>), dim3(nitems * 8), dim3(512), 0, st, la);
             break;
+        case 804: launch_dqueue<RHO, 4, true>(nitems, st, la); break;
+        case 802: launch_dqueue<RHO, 2, true>(nitems, st, la); break;
         default: hipLaunchKernelGGL((level_kernel<RHO, 2, true>), grid, block, 0, st, la); break;
         }
         return;
@@ -1051,6 +1113,8 @@ void launch_level(int variant, int nitems, hipStream_t st, const LevelArgs& la, 
     case 708:
         hipLaunchKernelGGL((level_kernel_dsplit<RHO, 8>), dim3(nitems * 8), dim3(512), 0, st, la);
         break;
+    case 804: launch_dqueue<RHO, 4, false>(nitems, st, la); break;
+    case 802: launch_dqueue<RHO, 2, false>(nitems, st, la); break;
     default: hipLaunchKernelGGL((level_kernel<RHO, 8>), grid, block, 0, st, la); break;
     }
 }
@@ -1483,6 +1547,7 @@ struct octofmm_plan {
     // the far offsets evaluated in FP32 as a whole (far_delta order)
     double* d_recm = nullptr;
     uint8_t* d_mixoff = nullptr;
+    uint32_t* d_queue = nullptr;   // per-XCD work counters of the persistent variants
     int64_t mixed_far_pairs_by_offset = 0;   // FP32 far pairs under that rule
     int64_t mixed_far_pairs_by_pair = 0;     // and under the per-pair rule of mapping 0
     bool root = false;             // root sub-grid: root stencil over slot 0
@@ -1811,6 +1876,7 @@ static octofmm_status launch_subgrid(octofmm_context* c, const SubgridWorkspace&
         la.mixbits = nullptr;
         la.recm = nullptr;
         la.mixoff = nullptr;
+        la.queue = nullptr;
         la.mon = w.mon;
         la.p2m_dir = c->d_p2m_dir;
         la.p2m_tgt = c->d_p2m_tgt;
@@ -2055,6 +2121,7 @@ static void plan_free(octofmm_plan* p) {
     (void)hipFree(p->d_mixbits);
     (void)hipFree(p->d_recm);
     (void)hipFree(p->d_mixoff);
+    (void)hipFree(p->d_queue);
     (void)hipFree(p->d_partial);
     (void)hipFree(p->d_pslots);
     (void)hipFree(p->d_pidx);
@@ -2124,6 +2191,7 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         la.mixbits = p->d_mixbits;
         la.recm = p->d_recm;
         la.mixoff = p->d_mixoff;
+        la.queue = p->d_queue;
         la.mon = p->d_mon;
         la.p2m_dir = c->d_p2m_dir;
         la.p2m_tgt = c->d_p2m_tgt;
@@ -2204,17 +2272,19 @@ static octofmm_status plan_run(octofmm_plan* p, hipStream_t st, int mask) {
 
 // Kernel variants a plan may run: FP64 plans every variant, mixed-precision
 // plans the mapping-0 forms (8, 4, 2 waves per workgroup; FP32 chosen per
-// pair) and mapping 7 (704, 702, 708; FP32 chosen per coarse offset).
+// pair) and mapping 7 with its persistent form (704, 702, 708, 804, 802;
+// FP32 chosen per coarse offset).
 static bool variant_allowed(const octofmm_plan* p, int v) {
     bool known = false;
     for (int k : kVariants) known = known || k == v;
-    return known && !(p->precision == OCTOFMM_PRECISION_MIXED && v >= 100 && v / 100 != 7);
+    const int mapping = v / 100;
+    return known && !(p->precision == OCTOFMM_PRECISION_MIXED && mapping != 0 && mapping != 7 && mapping != 8);
 }
 
 // The plan's count of FP32 far pairs follows the chosen variant's rule.
 static void set_mixed_count(octofmm_plan* p) {
     if (p->precision == OCTOFMM_PRECISION_MIXED)
-        p->stats.mixed_far_pairs = p->variant / 100 == 7 ? p->mixed_far_pairs_by_offset : p->mixed_far_pairs_by_pair;
+        p->stats.mixed_far_pairs = p->variant >= 700 ? p->mixed_far_pairs_by_offset : p->mixed_far_pairs_by_pair;
 }
 
 // Tuning cache: the variant the autotuner chose, keyed by level shape (device
@@ -2318,7 +2388,7 @@ static octofmm_status plan_autotune_run(octofmm_plan* p) {
     HIPC(c, hipEventCreate(&e1));
     float best = 1e30f;
     int best_w = 8;
-    for (int w : kVariants) {
+    for (int w : kTunedVariants) {
         if (!variant_allowed(p, w)) continue;
         p->variant = w;
         s = plan_launch(p, st, 2);   // warm-up
@@ -2551,6 +2621,8 @@ static octofmm_status level_plan_build(octofmm_context* c, const octofmm_level* 
         return cleanup(OCTOFMM_ERR_HIP, "plan table upload failed");
     if (root && hipMalloc((void**)&p->d_partial, sizeof(double) * kRootChunks * kCells * kRecDoubles) != hipSuccess)
         return cleanup(OCTOFMM_ERR_ALLOCATION, "root partial buffer allocation failed");
+    if (!root && hipMalloc((void**)&p->d_queue, sizeof(uint32_t) * 8) != hipSuccess)
+        return cleanup(OCTOFMM_ERR_ALLOCATION, "work counter allocation failed");
     if (!pslots.empty()) {
         if (hipMalloc((void**)&p->d_pidx, sizeof(int2) * std::max<size_t>(1, pidx.size())) != hipSuccess ||
             hipMalloc((void**)&p->d_pslots, sizeof(PartialSlot) * pslots.size()) != hipSuccess)

@@ -178,17 +178,19 @@ def test_level_config2_full_size(ctx):
 
 
 @pytest.mark.parametrize("t", [_abi.RHO, _abi.DRHODT])
-@pytest.mark.parametrize("variant", [8, 4, 2, 304, 302, 504, 704, 702, 708])
+@pytest.mark.parametrize("variant", [8, 4, 2, 304, 302, 504, 704, 702, 708, 804, 802])
 def test_level_kernel_variants(ctx, variant, t, monkeypatch):
     """Every lane mapping and workgroup shape of the level kernel gives the
     oracle's result (mixed level: M2L far + near passes, p2p, p2m), for both
-    solve types."""
+    solve types, on every replay of its graph (the persistent variants clear
+    their work counters in each one)."""
     monkeypatch.setenv("OCTOFMM_VARIANT", str(variant))
     lv = workloads.mixed_level((6, 6, 6), seed=13, leaf_fraction=0.5)
     print(f"variant {variant} type {t}", flush=True)
-    outs, stats = run_level(ctx, lv, t)
+    outs, stats = run_level(ctx, lv, t, repeats=2)
     assert stats["variant"] == variant
     L, Lc = outs[0]
+    assert np.array_equal(L, outs[1][0]) and np.array_equal(Lc, outs[1][1])
     Lr, Lcr = oracle.level_interactions(lv, type_=t)
     tg = lv.targets
     assert normwise(L[tg], Lr[tg]) <= TOL
@@ -197,7 +199,7 @@ def test_level_kernel_variants(ctx, variant, t, monkeypatch):
 
 
 @pytest.mark.parametrize("theta", [0.5, 0.34])
-@pytest.mark.parametrize("variant", [8, 304, 504, 704, 708])
+@pytest.mark.parametrize("variant", [8, 304, 504, 704, 708, 804])
 def test_level_edge_targets_missing_neighbours(theta, variant, monkeypatch):
     """Targets on the faces, edges and corners of the domain, whose partner
     coarse cells beyond it do not exist: every lane mapping must leave those
@@ -236,12 +238,12 @@ def test_autotuned_plan_leaves_outputs_untouched_until_executed(ctx):
 
 MIXED_FORCE_TOL = 1e-6      # total force and torque, relative to the oracle's totals
 MIXED_COMPONENT_TOL = 1e-5  # per component normwise
-MIXED_VARIANTS = [8, 4, 2, 704, 702, 708]
+MIXED_VARIANTS = [8, 4, 2, 704, 702, 708, 804, 802]
 # FP32 share of the far pairs at theta = 0.5, mixed_far_d2 = 4: mapping 0
 # chooses per pair (|d|^2 > 4: 183 of 189 per cell), mapping 7 per coarse
 # offset (the 20 edge and corner offsets: 153 of 189; the 6 face offsets,
 # which hold the |d| = 2 pairs, stay FP64)
-MIXED_SHARE = {8: 0.96, 4: 0.96, 2: 0.96, 704: 0.8, 702: 0.8, 708: 0.8}
+MIXED_SHARE = {8: 0.96, 4: 0.96, 2: 0.96, 704: 0.8, 702: 0.8, 708: 0.8, 804: 0.8, 802: 0.8}
 
 
 @pytest.mark.parametrize("t", [_abi.RHO, _abi.DRHODT])
