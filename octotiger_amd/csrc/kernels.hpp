@@ -444,13 +444,17 @@ __device__ __forceinline__ void m2l_near(double mn, double Y0, double Y1, double
     }
 }
 
-// One p2m pair: refined-neighbour multipole record p onto a leaf cell at X.
+// One p2m pair: refined-neighbour multipole record r (held in registers)
+// onto a leaf cell at X.
 template <bool RHO>
-__device__ __forceinline__ void p2m_pair(const double* __restrict__ p, double X0, double X1, double X2,
-    double* L, double* C) {
+__device__ __forceinline__ void p2m_pair_rec(const Rec& r, double X0, double X1, double X2, double* L, double* C) {
     double m[20];
-    load_record20(p, m);
-    const double x = X0 - p[20], y = X1 - p[21], z = X2 - p[22];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        m[2 * i] = r.v[i].x;
+        m[2 * i + 1] = r.v[i].y;
+    }
+    const double x = X0 - r.v[10].x, y = X1 - r.v[10].y, z = X2 - r.v[11].x;
     DTensor t;
     d_tensor(x, y, z, t);
 #pragma unroll
@@ -493,18 +497,9 @@ struct LevelArgs {
     const double* recm;
     const uint8_t* mixoff;
     const double* mon;      // [nslots][512] monopoles (p2p window of leaf targets)
-    // p2m pair lists by neighbour direction (raw 0..26): dir[2r] targets in
-    // direction r starting at dir[2r+1] of tgt; tgt = (target cell, first
-    // partner, partner count, 0); part = partner record index in the neighbour
-    const int32_t* p2m_dir;
-    const int4* p2m_tgt;
-    const int32_t* p2m_part;
-    // per direction, the union of its partner records (the slab staged in LDS
-    // on the theta = 0.5 path): [27][2] (count, offset), the record indices,
-    // and the partners as slab indices (same layout as p2m_part)
-    const int32_t* p2m_sdir;
-    const int32_t* p2m_slab;
-    const int32_t* p2m_spart;
+    // leaf targets with refined neighbours: the p2m sums (L0..3, L_c, 0) of
+    // each cell, [p2m item][512][8], written by p2m_kernel for leaf_kernel
+    double* p2m_out;
     // Whole-tree solves (l2l != 0): the level's expansions are written, not
     // accumulated, as interactions + the parent cell's final expansion
     // re-expanded about the cell's centre of mass (and L_c + the parent's

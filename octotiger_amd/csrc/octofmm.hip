@@ -725,9 +725,9 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
 // for every lane: each pair is one LDS read and four FMAs with scalar
 // operands (monopole_kernel_templates.hpp:16-23, kokkos_kernel.hpp:255-284;
 // the 1/dx and -1/dx^2 factors are applied once per cell).  Refined
-// neighbours then contribute through p2m with their full multipoles over the
-// same stencil, far and near alike (monopole_kernel_templates.hpp:69-239).
-constexpr int kP2mSlab = 128;   // records of one direction's p2m slab in LDS (theta = 0.5: 128)
+// neighbours contribute through p2m with their full multipoles over the same
+// stencil, far and near alike (monopole_kernel_templates.hpp:69-239): those
+// sums come from p2m_kernel, launched before this kernel on the same stream.
 
 template <bool RHO, bool CUBE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void leaf_kernel(LevelArgs a) {
@@ -814,113 +814,19 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
     double C[3] = {0.0, 0.0, 0.0};
     const bool p2m = (it.y & kItemP2M) != 0;
     if (p2m) {
-        // refined neighbours, one direction at a time: each listed target of
-        // the direction gets 8 lanes, which stride over its partners and
-        // combine their sums by a fixed butterfly; lane 0 of the group adds
-        // the result to the target's LDS slot.  A barrier separates
-        // directions, so every target's sum is formed in the same order on
-        // every run.
-        double* acc = s_win;   // the p2p window is no longer needed
-        // theta = 0.5: the direction's partner records (at most kP2mSlab: two
-        // fine layers of a face) are first copied into LDS by the whole
-        // workgroup, all loads in flight at once, and the pairs read them from
-        // there instead of waiting on one global load per pair
-        double* slab = s_win + kCells * 7;
-        __syncthreads();
-        for (int i = threadIdx.x; i < kCells * 7; i += 512) acc[i] = 0.0;
-        __syncthreads();
-        if constexpr (CUBE) {
-            // Opposite directions r and 26 - r reach disjoint target cells (the
-            // stencil reaches 2 cells past the sub-grid, less than half of it;
-            // the host checks this), so they share one phase: two slabs, one
-            // target list, half the barriers, and the few targets of an edge
-            // or corner direction fill more of the workgroup.  Each target
-            // still receives its directions' sums in a fixed order.
-            double* slab2 = slab + kP2mSlab * kRecDoubles;
-            for (int r = 0; r < 13; ++r) {
-                const int ra = r, rb = 26 - r;
-                const bool on_a = s_nbr[ra] >= 0 && s_kind[ra] == OCTOFMM_SLOT_NONLEAF;
-                const bool on_b = s_nbr[rb] >= 0 && s_kind[rb] == OCTOFMM_SLOT_NONLEAF;
-                if (!on_a && !on_b) continue;   // uniform
-                const int nta = on_a ? a.p2m_dir[2 * ra] : 0, ntb = on_b ? a.p2m_dir[2 * rb] : 0;
-                const int sna = on_a ? a.p2m_sdir[2 * ra] : 0, snb = on_b ? a.p2m_sdir[2 * rb] : 0;
-                const int soa = a.p2m_sdir[2 * ra + 1], sob = a.p2m_sdir[2 * rb + 1];
-                const double2* srca = reinterpret_cast<const double2*>(
-                    a.rec + (size_t)(on_a ? s_nbr[ra] : 0) * kCells * kRecDoubles);
-                const double2* srcb = reinterpret_cast<const double2*>(
-                    a.rec + (size_t)(on_b ? s_nbr[rb] : 0) * kCells * kRecDoubles);
-                double2* dsta = reinterpret_cast<double2*>(slab);
-                double2* dstb = reinterpret_cast<double2*>(slab2);
-                for (int c2 = threadIdx.x; c2 < (sna + snb) * 12; c2 += 512) {
-                    if (c2 < sna * 12)
-                        dsta[c2] = srca[(size_t)a.p2m_slab[soa + c2 / 12] * 12 + c2 % 12];
-                    else {
-                        const int c3 = c2 - sna * 12;
-                        dstb[c3] = srcb[(size_t)a.p2m_slab[sob + c3 / 12] * 12 + c3 % 12];
-                    }
-                }
-                __syncthreads();
-                const int t0a = a.p2m_dir[2 * ra + 1], t0b = a.p2m_dir[2 * rb + 1];
-                for (int q = threadIdx.x; q < (nta + ntb) * 8; q += 512) {
-                    const int sub = q & 7, ti = q >> 3;   // 8-lane groups never straddle the two lists
-                    const bool second = ti >= nta;
-                    const int4 te = a.p2m_tgt[second ? t0b + (ti - nta) : t0a + ti];
-                    const double* pb = second ? slab2 : slab;
-                    const double* xc = a.com + ((size_t)slot * kCells + te.x) * 3;
-                    const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
-                    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                    // the next partner's index is requested one pair ahead,
-                    // so its round trip overlaps the current pair
-                    const int32_t* ip = a.p2m_spart + te.y;
-                    int nxt = sub < te.z ? ip[sub] : 0;
-                    for (int e = sub; e < te.z; e += 8) {
-                        const int cur = nxt;
-                        if (e + 8 < te.z) nxt = ip[e + 8];
-                        p2m_pair<RHO>(pb + (size_t)cur * kRecDoubles, X0, X1, X2, v, v + 4);
-                    }
-#pragma unroll
-                    for (int i = 0; i < 7; ++i) v[i] = sum8(v[i]);
-                    if (sub == 0) {
-                        double* o = acc + te.x * 7;
-#pragma unroll
-                        for (int i = 0; i < 7; ++i) o[i] += v[i];
-                    }
-                }
-                __syncthreads();
-            }
-        } else
-        for (int r = 0; r < 27; ++r) {
-            const int ns = s_nbr[r];
-            if (r == 13 || ns < 0 || s_kind[r] != OCTOFMM_SLOT_NONLEAF) continue;
-            const int nt = a.p2m_dir[2 * r], t0 = a.p2m_dir[2 * r + 1];
-            const double* pbase = a.rec + (size_t)ns * kCells * kRecDoubles;
-            const int32_t* pidx = a.p2m_part;
-            for (int q = threadIdx.x; q < nt * 8; q += 512) {
-                const int sub = q & 7;
-                const int4 te = a.p2m_tgt[t0 + (q >> 3)];
-                const double* xc = a.com + ((size_t)slot * kCells + te.x) * 3;
-                const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
-                double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                for (int e = sub; e < te.z; e += 8)
-                    p2m_pair<RHO>(pbase + (size_t)pidx[te.y + e] * kRecDoubles, X0, X1, X2, v, v + 4);
-#pragma unroll
-                for (int i = 0; i < 7; ++i) v[i] = sum8(v[i]);
-                if (sub == 0) {
-                    double* o = acc + te.x * 7;
-#pragma unroll
-                    for (int i = 0; i < 7; ++i) o[i] += v[i];
-                }
-            }
-            __syncthreads();
-        }
-        const double* o = acc + cell * 7;
-        L[0] += o[0];
-        L[1] += o[1];
-        L[2] += o[2];
-        L[3] += o[3];
-        C[0] = o[4];
-        C[1] = o[5];
-        C[2] = o[6];
+        // refined neighbours: the p2m sums p2m_kernel formed for this leaf
+        // (leaf items with p2m come first among the leaf items, so the item's
+        // place among them indexes p2m_out)
+        const double2* ps = reinterpret_cast<const double2*>(
+            a.p2m_out + ((size_t)xcd_item(blockIdx.x, gridDim.x) * kCells + cell) * 8);
+        const double2 v0 = ps[0], v1 = ps[1], v2 = ps[2], v3 = ps[3];
+        L[0] += v0.x;
+        L[1] += v0.y;
+        L[2] += v1.x;
+        L[3] += v1.y;
+        C[0] = v2.x;
+        C[1] = v2.y;
+        C[2] = v3.x;
     }
     if constexpr (RHO) {
         if (a.l2l && a.g) {   // uniform: whole-tree RHO solve
@@ -934,14 +840,175 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
     finish_cell<RHO>(a, slot, cell, L, 4, C, p2m, true);
 }
 
+// p2m on the staging of lane mapping 7: for leaf targets with refined
+// neighbours, the pairs whose partner cell lies in a refined neighbour slot
+// (far and near alike, monopole_kernel_templates.hpp:69-239) are the M2L pair
+// set of the target's cells restricted to those slots.  So a workgroup of
+// SPLIT waves takes one coarse block of one such leaf (lane = coarse cell x
+// child, as in the M2L kernels) and walks only the coarse offsets that reach
+// a refined neighbour from that block; each offset's 64 partner records are
+// staged into the wave's LDS image by direct loads and read as 8-lane
+// broadcasts.  The waves split the offsets and hand their sums to wave 0,
+// which adds them in wave order and writes each cell's (L0..3, L_c, 0) to
+// p2m_out[p2m item][cell].
+template <bool RHO, int SPLIT>
+__device__ void p2m_item(const LevelArgs& a, int slot, int ordinal, int w, const int* s_nbr, const int* s_kind,
+    const int* s_live, int nlive, double2* __restrict__ s_stage, int sl) {
+    double2* stage = s_stage + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * kDirWave;
+    const int l = threadIdx.x & 63, c = l >> 3;
+    const auto* tab = as_const(a.tab);
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const LaneGeom g = lane_geom(wu);
+    const double* xc = a.com + ((size_t)slot * kCells + g.cell) * 3;
+    const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
+    double L[4] = {0.0, 0.0, 0.0, 0.0}, C[3] = {0.0, 0.0, 0.0};
+    const int bx = ((wu >> 2) & 1) * 2, by = ((wu >> 1) & 1) * 2, bz = (wu & 1) * 2;
+    // partner coarse cell of group gi at offset (ax, ay, az): its records, and
+    // bit gi of `absent` where it is not in a refined neighbour slot
+    auto group_base = [&](int gi, int ax, int ay, int az, unsigned& absent) -> const char* {
+        const int px = bx + ((gi >> 2) & 1) + ax, py = by + ((gi >> 1) & 1) + ay, pz = bz + (gi & 1) + az;
+        const int raw = (px < 0 ? 0 : (px < 4 ? 1 : 2)) + 3 * (py < 0 ? 0 : (py < 4 ? 1 : 2)) +
+            9 * (pz < 0 ? 0 : (pz < 4 ? 1 : 2));
+        const int ns = s_nbr[raw];
+        const bool refined = raw != 13 && ns >= 0 && s_kind[raw] == OCTOFMM_SLOT_NONLEAF;
+        if (!refined) absent |= 1u << gi;
+        const int cb = ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
+        return reinterpret_cast<const char*>(a.rec + ((size_t)(refined ? ns : 0) * kCells + cb) * kRecDoubles);
+    };
+    const unsigned loff = (unsigned)l * 16u;
+    const char* nbase[8];
+    unsigned absent_next = 0;
+    auto bases = [&](int di) {
+        const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+        unsigned absent = 0;
+#pragma unroll
+        for (int gi = 0; gi < 8; ++gi) nbase[gi] = group_base(gi, ax, ay, az, absent);
+        asm volatile("" ::"s"(nbase[0]), "s"(nbase[1]), "s"(nbase[2]), "s"(nbase[3]), "s"(nbase[4]),
+                     "s"(nbase[5]), "s"(nbase[6]), "s"(nbase[7]));
+        absent_next = absent;
+    };
+    // the full-wave loads of every group first, the half-wave loads after
+    // them in a branch of their own (as in m2l_item_direct)
+    auto issue = [&]() {
+#pragma unroll
+        for (int gi = 0; gi < 8; ++gi)
+            __builtin_amdgcn_global_load_lds(nbase[gi] + loff, (lds_ptr)(stage + gi * kDirGroup), 16, 0, 0);
+        if (l < 32) {
+#pragma unroll
+            for (int gi = 0; gi < 8; ++gi)
+                __builtin_amdgcn_global_load_lds(nbase[gi] + 1024 + loff, (lds_ptr)(stage + gi * kDirGroup + 64), 16, 0, 0);
+        }
+    };
+    const double2* group = stage + c * kDirGroup;
+    const int q0 = __builtin_amdgcn_readfirstlane(sl);
+    if (q0 < nlive) {
+        bases(s_live[q0]);
+        issue();
+    }
+    for (int q = q0; q < nlive; q += SPLIT) {
+        const int di = s_live[q];
+        const unsigned all = (unsigned)((tab->farbits[di] | tab->nearbits[di]) >> (8 * (l & 7))) & 0xffu;
+        const unsigned pb = ((absent_next >> c) & 1u) ? 0u : all;
+        if (q + SPLIT < nlive) bases(s_live[q + SPLIT]);
+        __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): this wave's image is in LDS
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (pb & (1u << j)) {
+                Rec cur;
+#pragma unroll
+                for (int i = 0; i < 12; ++i) cur.v[i] = group[j * 12 + i];
+                p2m_pair_rec<RHO>(cur, X0, X1, X2, L, C);
+            }
+        }
+        lds_wave_sync();   // every read of the image done before it is refilled
+        if (q + SPLIT < nlive) issue();
+    }
+    // waves 1..SPLIT-1 hand their sums to wave 0 through their own images
+    double* part = reinterpret_cast<double*>(stage) + l * 7;
+    if (q0 != 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part[i] = L[i];
+        part[4] = C[0];
+        part[5] = C[1];
+        part[6] = C[2];
+    }
+    __syncthreads();
+    if (q0 != 0) return;
+    for (int o = 1; o < SPLIT; ++o) {
+        const double* po = reinterpret_cast<const double*>(s_stage + o * kDirWave) + l * 7;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) L[i] += po[i];
+        C[0] += po[4];
+        C[1] += po[5];
+        C[2] += po[6];
+    }
+    double2* out = reinterpret_cast<double2*>(a.p2m_out + ((size_t)ordinal * kCells + g.cell) * 8);
+    out[0] = make_double2(L[0], L[1]);
+    out[1] = make_double2(L[2], L[3]);
+    out[2] = make_double2(C[0], C[1]);
+    out[3] = make_double2(C[2], 0.0);
+}
+
+// One workgroup of SPLIT waves per (leaf item with p2m, coarse block); the
+// items are a.items[a.item0 ...], in the order their sums are stored.
+template <bool RHO, int SPLIT>
+__global__ __launch_bounds__(64 * SPLIT) void p2m_kernel(LevelArgs a) {
+    __shared__ int s_nbr[27];
+    __shared__ int s_kind[27];
+    __shared__ int s_live[kMaxDelta];
+    __shared__ int s_nlive;
+    __shared__ double2 s_stage[SPLIT * kDirWave];
+    const int v = xcd_item(blockIdx.x, gridDim.x);
+    const int ordinal = v >> 3, w = v & 7;
+    const int2 it = a.items[a.item0 + ordinal];
+    if (threadIdx.x < 27) {
+        const int n = a.nbr[(size_t)it.x * 27 + threadIdx.x];
+        s_nbr[threadIdx.x] = n;
+        s_kind[threadIdx.x] = n >= 0 ? a.kind[n] : -1;
+    }
+    __syncthreads();
+    // the coarse offsets with a pair whose partner coarse cell, seen from this
+    // block, lies in a refined neighbour slot (wave 0, one offset per lane,
+    // compacted in offset order by a ballot)
+    if (threadIdx.x < 64) {
+        const Tables* tab = a.tab;
+        const int lane = threadIdx.x;
+        const int bx = ((w >> 2) & 1) * 2, by = ((w >> 1) & 1) * 2, bz = (w & 1) * 2;
+        int n = 0;
+        for (int base = 0; base < tab->ndelta; base += 64) {
+            const int di = base + lane;
+            bool live = false;
+            if (di < tab->ndelta && (tab->farbits[di] | tab->nearbits[di])) {
+                const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+                for (int gi = 0; gi < 8; ++gi) {
+                    const int px = bx + ((gi >> 2) & 1) + ax, py = by + ((gi >> 1) & 1) + ay, pz = bz + (gi & 1) + az;
+                    const int raw = (px < 0 ? 0 : (px < 4 ? 1 : 2)) + 3 * (py < 0 ? 0 : (py < 4 ? 1 : 2)) +
+                        9 * (pz < 0 ? 0 : (pz < 4 ? 1 : 2));
+                    live = live || (raw != 13 && s_nbr[raw] >= 0 && s_kind[raw] == OCTOFMM_SLOT_NONLEAF);
+                }
+            }
+            const uint64_t m = __ballot(live);
+            if (live) s_live[n + __popcll(m & ((uint64_t(1) << lane) - 1))] = di;
+            n += __popcll(m);
+        }
+        if (lane == 0) s_nlive = n;
+    }
+    __syncthreads();
+    p2m_item<RHO, SPLIT>(a, it.x, ordinal, w, s_nbr, s_kind, s_live, s_nlive, s_stage, (int)(threadIdx.x >> 6));
+}
+
 template <bool RHO>
-void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, const Tables& tab, int slab_max, bool opp_disjoint) {
+void launch_p2m(int nitems, hipStream_t st, const LevelArgs& la) {
+    hipLaunchKernelGGL((p2m_kernel<RHO, 4>), dim3(nitems * 8), dim3(256), 0, st, la);
+}
+
+template <bool RHO>
+void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, const Tables& tab) {
     const int W = kInx + 2 * tab.p2p_halo;
-    const bool cube = tab.p2p_cube && slab_max <= kP2mSlab && opp_disjoint;
-    // window, p2m sums (512 x 7) with two p2m slabs on the cube path, and the
-    // staged L rows of half a sub-grid (256 x 20)
-    const size_t p2m = (size_t)kCells * 7 + (cube ? (size_t)2 * kP2mSlab * kRecDoubles : 0);
-    const size_t lds = sizeof(double) * std::max<size_t>(std::max<size_t>((size_t)W * W * W, p2m), (size_t)256 * 20);
+    const bool cube = tab.p2p_cube != 0;
+    // window, and the staged L rows of half a sub-grid (256 x 20)
+    const size_t lds = sizeof(double) * std::max<size_t>((size_t)W * W * W, (size_t)256 * 20);
     if (cube)
         hipLaunchKernelGGL((leaf_kernel<RHO, true>), dim3(nitems), dim3(512), lds, st, la);
     else
@@ -1425,6 +1492,7 @@ struct SubgridWorkspace {
     int2* cidx = nullptr;       // [26][512] (cell, position in the handed-over arrays)
     int32_t* meta = nullptr;    // kMetaSize
     double* fx = nullptr;       // [27][3]
+    double* p2m = nullptr;      // [512][8] p2m sums of a leaf target
 };
 
 constexpr size_t ws_align(size_t n) { return (n + 255) & ~size_t(255); }
@@ -1441,7 +1509,8 @@ constexpr size_t kWsCMon = kWsCCom + ws_align(sizeof(double) * 26 * kCells * 3);
 constexpr size_t kWsCIdx = kWsCMon + ws_align(sizeof(double) * 26 * kCells);
 constexpr size_t kWsMeta = kWsCIdx + ws_align(sizeof(int2) * 26 * kCells);
 constexpr size_t kWsFx = kWsMeta + ws_align(sizeof(int32_t) * kMetaSize);
-constexpr size_t kWsBytes = kWsFx + ws_align(sizeof(double) * 81);
+constexpr size_t kWsP2m = kWsFx + ws_align(sizeof(double) * 81);
+constexpr size_t kWsBytes = kWsP2m + ws_align(sizeof(double) * kCells * 8);
 
 SubgridWorkspace carve_workspace(char* base) {
     SubgridWorkspace w;
@@ -1458,6 +1527,7 @@ SubgridWorkspace carve_workspace(char* base) {
     w.cidx = reinterpret_cast<int2*>(base + kWsCIdx);
     w.meta = reinterpret_cast<int32_t*>(base + kWsMeta);
     w.fx = reinterpret_cast<double*>(base + kWsFx);
+    w.p2m = reinterpret_cast<double*>(base + kWsP2m);
     return w;
 }
 
@@ -1509,15 +1579,6 @@ struct octofmm_context {
     int32_t* h_meta = nullptr;     // items[2] nbr[27*27] kind[27] foff[27*3]
     double* h_fx = nullptr;        // [27][3]
     int2* h_idx = nullptr;         // [26][512] (cell, position) of partial halos
-    // p2m pair lists by direction (built from the tables at creation)
-    int32_t* d_p2m_dir = nullptr;
-    int4* d_p2m_tgt = nullptr;
-    int32_t* d_p2m_part = nullptr;
-    int32_t* d_p2m_sdir = nullptr;
-    int32_t* d_p2m_slab = nullptr;
-    int32_t* d_p2m_spart = nullptr;
-    int32_t p2m_slab_max = 0;      // largest per-direction slab (records)
-    bool p2m_opp_disjoint = false; // opposite directions reach disjoint target cells
 };
 
 struct octofmm_plan {
@@ -1548,6 +1609,8 @@ struct octofmm_plan {
     double* d_recm = nullptr;
     uint8_t* d_mixoff = nullptr;
     uint32_t* d_queue = nullptr;   // per-XCD work counters of the persistent variants
+    int32_t n_p2m = 0;             // leaf items with p2m: items [n_m2l, n_m2l + n_p2m)
+    double* d_p2m_out = nullptr;   // their p2m sums [n_p2m][512][8]
     int64_t mixed_far_pairs_by_offset = 0;   // FP32 far pairs under that rule
     int64_t mixed_far_pairs_by_pair = 0;     // and under the per-pair rule of mapping 0
     bool root = false;             // root sub-grid: root stencil over slot 0
@@ -1747,73 +1810,6 @@ octofmm_status load_subgrid(octofmm_context* c, const SubgridWorkspace& w, const
     return OCTOFMM_OK;
 }
 
-// p2m pair lists: for each neighbour direction, the target cells whose
-// stencil reaches into it and, per target, the partner records there, in
-// (coarse offset, child) order.
-void build_p2m_lists(const Tables& T, std::vector<int32_t>& dir, std::vector<int4>& tgt,
-    std::vector<int32_t>& part, std::vector<int32_t>& sdir, std::vector<int32_t>& slab,
-    std::vector<int32_t>& spart, int& slab_max, bool& opp_disjoint) {
-    std::vector<int32_t> lists[27][kCells];
-    for (int t = 0; t < kCells; ++t) {
-        const int x = t >> 6, y = (t >> 3) & 7, z = t & 7;
-        const int k = child_of(x, y, z);
-        for (int half = 0; half < 2; ++half) {
-            const uint16_t* pl = half ? T.nearlist[k] : T.farlist[k];
-            const int np = half ? T.nnear_k[k] : T.nfar_k[k];
-            for (int q = 0; q < np; ++q) {
-                const int di = pl[q] >> 3, j = pl[q] & 7;
-                const int px = 2 * ((x >> 1) + T.delta[di][0]) + ((j >> 2) & 1);
-                const int py = 2 * ((y >> 1) + T.delta[di][1]) + ((j >> 1) & 1);
-                const int pz = 2 * ((z >> 1) + T.delta[di][2]) + (j & 1);
-                const int r = (px < 0 ? 0 : px < kInx ? 1 : 2) + 3 * (py < 0 ? 0 : py < kInx ? 1 : 2) +
-                    9 * (pz < 0 ? 0 : pz < kInx ? 1 : 2);
-                if (r == 13) continue;
-                lists[r][t].push_back(record_of_cell(px & 7, py & 7, pz & 7));
-            }
-        }
-    }
-    dir.assign(54, 0);
-    tgt.clear();
-    part.clear();
-    for (int r = 0; r < 27; ++r) {
-        dir[2 * r + 1] = (int32_t)tgt.size();
-        for (int t = 0; t < kCells; ++t) {
-            if (lists[r][t].empty()) continue;
-            tgt.push_back(make_int4(t, (int)part.size(), (int)lists[r][t].size(), 0));
-            part.insert(part.end(), lists[r][t].begin(), lists[r][t].end());
-        }
-        dir[2 * r] = (int32_t)tgt.size() - dir[2 * r + 1];
-    }
-    // per direction, the union of the partner records, ascending, and every
-    // partner re-indexed into it
-    sdir.assign(54, 0);
-    slab.clear();
-    spart.assign(part.size(), 0);
-    slab_max = 0;
-    for (int r = 0; r < 27; ++r) {
-        std::vector<int> pos(kCells, -1);
-        for (int t = 0; t < kCells; ++t)
-            for (int rec : lists[r][t]) pos[rec] = 0;
-        sdir[2 * r + 1] = (int32_t)slab.size();
-        int n = 0;
-        for (int rec = 0; rec < kCells; ++rec)
-            if (pos[rec] == 0) {
-                pos[rec] = n++;
-                slab.push_back(rec);
-            }
-        sdir[2 * r] = n;
-        slab_max = std::max(slab_max, n);
-        for (int q = dir[2 * r + 1]; q < dir[2 * r + 1] + dir[2 * r]; ++q)
-            for (int e = 0; e < tgt[q].z; ++e) spart[tgt[q].y + e] = pos[part[tgt[q].y + e]];
-    }
-    // the leaf kernel's theta = 0.5 path runs directions r and 26 - r in one
-    // phase, which needs their target cells disjoint
-    opp_disjoint = true;
-    for (int r = 0; r < 13; ++r)
-        for (int t = 0; t < kCells; ++t)
-            if (!lists[r][t].empty() && !lists[26 - r][t].empty()) opp_disjoint = false;
-}
-
 }   // namespace
 
 // Runs one loaded sub-grid through a workspace: the caller's L is copied in
@@ -1878,12 +1874,7 @@ static octofmm_status launch_subgrid(octofmm_context* c, const SubgridWorkspace&
         la.mixoff = nullptr;
         la.queue = nullptr;
         la.mon = w.mon;
-        la.p2m_dir = c->d_p2m_dir;
-        la.p2m_tgt = c->d_p2m_tgt;
-        la.p2m_part = c->d_p2m_part;
-        la.p2m_sdir = c->d_p2m_sdir;
-        la.p2m_slab = c->d_p2m_slab;
-        la.p2m_spart = c->d_p2m_spart;
+        la.p2m_out = w.p2m;
         la.parent = nullptr;
         la.octant = nullptr;
         la.pL = la.pLc = la.pcom = nullptr;
@@ -1897,10 +1888,17 @@ static octofmm_status launch_subgrid(octofmm_context* c, const SubgridWorkspace&
         la.inv_dx = 1.0 / dx;
         la.neg_inv_dx = -1.0 / dx;
         if (item_flags & kItemLeaf) {
+            if (item_flags & kItemP2M) {
+                if (rho)
+                    launch_p2m<true>(1, st, la);
+                else
+                    launch_p2m<false>(1, st, la);
+                HIPC(c, hipGetLastError());
+            }
             if (rho)
-                launch_leaf<true>(1, st, la, c->tab, c->p2m_slab_max, c->p2m_opp_disjoint);
+                launch_leaf<true>(1, st, la, c->tab);
             else
-                launch_leaf<false>(1, st, la, c->tab, c->p2m_slab_max, c->p2m_opp_disjoint);
+                launch_leaf<false>(1, st, la, c->tab);
         } else if (rho) {
             launch_level<true>(8, 1, st, la);
         } else {
@@ -1967,28 +1965,6 @@ octofmm_status octofmm_create(const octofmm_config* cfg, octofmm_context** out) 
         octofmm_destroy(c);
         return OCTOFMM_ERR_HIP;
     }
-    std::vector<int32_t> pdir, ppart, psdir, pslab, pspart;
-    std::vector<int4> ptgt;
-    build_p2m_lists(c->tab, pdir, ptgt, ppart, psdir, pslab, pspart, c->p2m_slab_max, c->p2m_opp_disjoint);
-    if (hipMalloc((void**)&c->d_p2m_dir, sizeof(int32_t) * pdir.size()) != hipSuccess ||
-        hipMalloc((void**)&c->d_p2m_tgt, sizeof(int4) * std::max<size_t>(1, ptgt.size())) != hipSuccess ||
-        hipMalloc((void**)&c->d_p2m_part, sizeof(int32_t) * std::max<size_t>(1, ppart.size())) != hipSuccess ||
-        hipMemcpy(c->d_p2m_dir, pdir.data(), sizeof(int32_t) * pdir.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        (!ptgt.empty() &&
-            hipMemcpy(c->d_p2m_tgt, ptgt.data(), sizeof(int4) * ptgt.size(), hipMemcpyHostToDevice) != hipSuccess) ||
-        (!ppart.empty() &&
-            hipMemcpy(c->d_p2m_part, ppart.data(), sizeof(int32_t) * ppart.size(), hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMalloc((void**)&c->d_p2m_sdir, sizeof(int32_t) * psdir.size()) != hipSuccess ||
-        hipMalloc((void**)&c->d_p2m_slab, sizeof(int32_t) * std::max<size_t>(1, pslab.size())) != hipSuccess ||
-        hipMalloc((void**)&c->d_p2m_spart, sizeof(int32_t) * std::max<size_t>(1, pspart.size())) != hipSuccess ||
-        hipMemcpy(c->d_p2m_sdir, psdir.data(), sizeof(int32_t) * psdir.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        (!pslab.empty() &&
-            hipMemcpy(c->d_p2m_slab, pslab.data(), sizeof(int32_t) * pslab.size(), hipMemcpyHostToDevice) != hipSuccess) ||
-        (!pspart.empty() &&
-            hipMemcpy(c->d_p2m_spart, pspart.data(), sizeof(int32_t) * pspart.size(), hipMemcpyHostToDevice) != hipSuccess)) {
-        octofmm_destroy(c);
-        return OCTOFMM_ERR_HIP;
-    }
     *out = c;
     return OCTOFMM_OK;
 }
@@ -2002,12 +1978,6 @@ octofmm_status octofmm_destroy(octofmm_context* c) {
     delete[] c->h_idx;
     (void)hipFree(c->d_ws);
     (void)hipFree(c->d_tab);
-    (void)hipFree(c->d_p2m_dir);
-    (void)hipFree(c->d_p2m_tgt);
-    (void)hipFree(c->d_p2m_part);
-    (void)hipFree(c->d_p2m_sdir);
-    (void)hipFree(c->d_p2m_slab);
-    (void)hipFree(c->d_p2m_spart);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OCTOFMM_OK;
@@ -2122,6 +2092,7 @@ static void plan_free(octofmm_plan* p) {
     (void)hipFree(p->d_recm);
     (void)hipFree(p->d_mixoff);
     (void)hipFree(p->d_queue);
+    (void)hipFree(p->d_p2m_out);
     (void)hipFree(p->d_partial);
     (void)hipFree(p->d_pslots);
     (void)hipFree(p->d_pidx);
@@ -2193,12 +2164,7 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         la.mixoff = p->d_mixoff;
         la.queue = p->d_queue;
         la.mon = p->d_mon;
-        la.p2m_dir = c->d_p2m_dir;
-        la.p2m_tgt = c->d_p2m_tgt;
-        la.p2m_part = c->d_p2m_part;
-        la.p2m_sdir = c->d_p2m_sdir;
-        la.p2m_slab = c->d_p2m_slab;
-        la.p2m_spart = c->d_p2m_spart;
+        la.p2m_out = p->d_p2m_out;
         la.parent = p->t_parent;
         la.octant = p->t_octant;
         la.pL = p->t_pL;
@@ -2224,10 +2190,18 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         if ((stage_mask & 4) && p->nitems > p->n_m2l) {
             la.nitems = p->nitems - p->n_m2l;
             la.item0 = p->n_m2l;
+            if (p->n_p2m > 0) {
+                // the leaf items with p2m lead the leaf items
+                if (rho)
+                    launch_p2m<true>(p->n_p2m, st, la);
+                else
+                    launch_p2m<false>(p->n_p2m, st, la);
+                HIPC(c, hipGetLastError());
+            }
             if (rho)
-                launch_leaf<true>(la.nitems, st, la, c->tab, c->p2m_slab_max, c->p2m_opp_disjoint);
+                launch_leaf<true>(la.nitems, st, la, c->tab);
             else
-                launch_leaf<false>(la.nitems, st, la, c->tab, c->p2m_slab_max, c->p2m_opp_disjoint);
+                launch_leaf<false>(la.nitems, st, la, c->tab);
             HIPC(c, hipGetLastError());
         }
     }
@@ -2623,6 +2597,10 @@ static octofmm_status level_plan_build(octofmm_context* c, const octofmm_level* 
         return cleanup(OCTOFMM_ERR_ALLOCATION, "root partial buffer allocation failed");
     if (!root && hipMalloc((void**)&p->d_queue, sizeof(uint32_t) * 8) != hipSuccess)
         return cleanup(OCTOFMM_ERR_ALLOCATION, "work counter allocation failed");
+    p->n_p2m = (int32_t)buckets[2].size();
+    if (p->n_p2m > 0 &&
+        hipMalloc((void**)&p->d_p2m_out, sizeof(double) * (size_t)p->n_p2m * kCells * 8) != hipSuccess)
+        return cleanup(OCTOFMM_ERR_ALLOCATION, "p2m sum buffer allocation failed");
     if (!pslots.empty()) {
         if (hipMalloc((void**)&p->d_pidx, sizeof(int2) * std::max<size_t>(1, pidx.size())) != hipSuccess ||
             hipMalloc((void**)&p->d_pslots, sizeof(PartialSlot) * pslots.size()) != hipSuccess)
