@@ -2102,7 +2102,10 @@ static void plan_free(octofmm_plan* p) {
 
 // stage_mask bit 0: stage the partner records; bit 1: the root kernel, or the
 // level kernel over the multipole-target items; bit 2: the leaf kernel over
-// the monopole-target items.  Bits 1 and 2 write disjoint sub-grids.
+// the monopole-target items; bit 3: the p2m kernel of the leaf items with
+// refined neighbours, whose sums the leaf kernel reads (so bit 3 goes before
+// bit 2, on the same stream or ordered before it).  Bits 1 and 2 write
+// disjoint sub-grids.
 static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mask) {
     octofmm_context* c = p->ctx;
     if ((stage_mask & 1) && p->npartial > 0) {
@@ -2149,7 +2152,7 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
         rr.overwrite = p->l2l ? 1 : 0;
         hipLaunchKernelGGL(root_reduce, dim3((kCells * 23 + 255) / 256), dim3(256), 0, st, rr);
         HIPC(c, hipGetLastError());
-    } else if (!p->root && (stage_mask & 6) && p->nitems > 0) {
+    } else if (!p->root && (stage_mask & 14) && p->nitems > 0) {
         LevelArgs la;
         la.rec = p->d_rec;
         la.com = p->d_com;
@@ -2187,17 +2190,19 @@ static octofmm_status plan_launch(octofmm_plan* p, hipStream_t st, int stage_mas
                 launch_level<false>(p->variant, p->n_m2l, st, la, p->precision == OCTOFMM_PRECISION_MIXED);
             HIPC(c, hipGetLastError());
         }
+        if ((stage_mask & 8) && p->n_p2m > 0) {
+            // the leaf items with p2m lead the leaf items
+            la.nitems = p->n_p2m;
+            la.item0 = p->n_m2l;
+            if (rho)
+                launch_p2m<true>(p->n_p2m, st, la);
+            else
+                launch_p2m<false>(p->n_p2m, st, la);
+            HIPC(c, hipGetLastError());
+        }
         if ((stage_mask & 4) && p->nitems > p->n_m2l) {
             la.nitems = p->nitems - p->n_m2l;
             la.item0 = p->n_m2l;
-            if (p->n_p2m > 0) {
-                // the leaf items with p2m lead the leaf items
-                if (rho)
-                    launch_p2m<true>(p->n_p2m, st, la);
-                else
-                    launch_p2m<false>(p->n_p2m, st, la);
-                HIPC(c, hipGetLastError());
-            }
             if (rho)
                 launch_leaf<true>(la.nitems, st, la, c->tab);
             else
@@ -2219,7 +2224,7 @@ static bool plan_forks(const octofmm_plan* p, int mask) {
 
 static octofmm_status plan_fork_resources(octofmm_plan* p) {
     octofmm_context* c = p->ctx;
-    if (p->side || !plan_forks(p, 7)) return OCTOFMM_OK;
+    if (p->side || !plan_forks(p, 15)) return OCTOFMM_OK;
     HIPC(c, hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
     HIPC(c, hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
     HIPC(c, hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
@@ -2233,7 +2238,7 @@ static octofmm_status plan_run(octofmm_plan* p, hipStream_t st, int mask) {
     if (s != OCTOFMM_OK) return s;
     HIPC(c, hipEventRecord(p->ev_fork, st));
     HIPC(c, hipStreamWaitEvent(p->side, p->ev_fork, 0));
-    s = plan_launch(p, p->side, 4);
+    s = plan_launch(p, p->side, mask & 12);
     if (s != OCTOFMM_OK) return s;
     s = plan_launch(p, st, 2);
     if (s != OCTOFMM_OK) return s;
@@ -2714,7 +2719,7 @@ static octofmm_status level_plan_finish(octofmm_plan* p, int flags) {
             plan_free(p);
             return fail(c, OCTOFMM_ERR_HIP, "graph capture failed to start");
         }
-        octofmm_status ls = plan_run(p, cs, 7);
+        octofmm_status ls = plan_run(p, cs, 15);
         hipGraph_t g = nullptr;
         const hipError_t ee = hipStreamEndCapture(cs, &g);
         if (ls != OCTOFMM_OK || ee != hipSuccess) {
@@ -2758,7 +2763,7 @@ octofmm_status octofmm_level_plan_execute(octofmm_plan* p, void* stream) {
     }
     const octofmm_status fs = plan_fork_resources(p);   // first eager execute
     if (fs != OCTOFMM_OK) return fs;
-    return plan_run(p, st, 7);
+    return plan_run(p, st, 15);
 }
 
 octofmm_status octofmm_level_plan_stats(const octofmm_plan* p, octofmm_plan_stats* s) {
@@ -2781,7 +2786,7 @@ octofmm_status octofmm_level_plan_time_stages(octofmm_plan* p, void* stream, int
     for (int stage = 0; stage < 2; ++stage) {
         HIPC(c, hipEventRecord(e0, st));
         for (int i = 0; i < iters; ++i) {
-            octofmm_status s = plan_run(p, st, stage == 0 ? 1 : 6);
+            octofmm_status s = plan_run(p, st, stage == 0 ? 1 : 14);
             if (s != OCTOFMM_OK) return s;
         }
         HIPC(c, hipEventRecord(e1, st));
@@ -2914,8 +2919,12 @@ octofmm_status tree_launch(octofmm_tree_plan* t, hipStream_t st, int phases) {
             if (s != OCTOFMM_OK) return s;
             HIPC(c, hipEventRecord(t->ev_level[l], st));
             if (!p->root && p->nitems > p->n_m2l) {
+                // the p2m sums need only this level's records; the leaf
+                // kernel also needs the finished level above
                 hipStream_t sl = t->side_leaf[l & 1];
                 HIPC(c, hipStreamWaitEvent(sl, t->ev_stage[l], 0));
+                s = plan_launch(p, sl, 8);
+                if (s != OCTOFMM_OK) return s;
                 if (l > 0) HIPC(c, hipStreamWaitEvent(sl, t->ev_level[l - 1], 0));
                 s = plan_launch(p, sl, 4);
                 if (s != OCTOFMM_OK) return s;
