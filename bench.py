@@ -215,6 +215,11 @@ def run_tree(ctx, args, hip, _abi, np):
                      "stats": plan.stats(), "g": g, "plan_create_s": create_s,
                      "production_step": {"drhodt_ms": msd, "drhodt_then_rho_ms": ms2}}
         plan.close()
+    # the FP64 solve with leaf cells writing g (and L_c) only, no L rows
+    # (OCTOFMM_TREE_LEAF_G_ONLY)
+    plan = dev.plan(ctx, flags=flags | _abi.TREE_LEAF_G_ONLY)
+    res["fp64"]["leaf_g_only_ms_per_solve"] = time_plans([plan], stream, steps, max(1, args.warmup), hip)
+    plan.close()
     dev.free()
     # Octo-Tiger's own end-to-end regression value: the solid-sphere run
     # (test_problems/sphere/CMakeLists.txt:18) prints "pot 4.470774e-01 3.176879e+00"
@@ -243,6 +248,7 @@ def run_tree(ctx, args, hip, _abi, np):
         "workload": f"{t.name}: n=1.5 polytrope (radius 0.3, 1% log-normal noise), {t.nnodes} sub-grids, "
                     f"nodes per level {t.info['nodes_per_level']}, leaves per level {t.info['leaves_per_level']}",
         "fp64": {"ms_per_solve": res["fp64"]["ms_per_solve"], "phases_ms": res["fp64"]["phases_ms"],
+                 "leaf_g_only_ms_per_solve": res["fp64"]["leaf_g_only_ms_per_solve"],
                  "cell_interactions_per_s": pairs / (res["fp64"]["ms_per_solve"] * 1e-3),
                  "interaction_valu_frac": st["flops"] / (inter * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
                  "plan_create_s": res["fp64"]["plan_create_s"],
