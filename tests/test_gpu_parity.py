@@ -278,6 +278,36 @@ def test_mixed_precision_far_field(ctx, t, variant, monkeypatch):
         assert normwise(Lc[tg], Lcr[tg]) <= TOL
 
 
+@pytest.mark.parametrize("variant", [8, 704])
+def test_mixed_precision_other_theta(variant, monkeypatch):
+    """Mixed precision on the wider stencil of theta = 0.34 (coarse offsets
+    up to 2: 124 offsets, most of whose far pairs lie beyond 2 cells), per pair
+    (mapping 0) and per coarse offset (mapping 7), against the FP64 oracle."""
+    monkeypatch.setenv("OCTOFMM_VARIANT", str(variant))
+    c = native.Context(theta=0.34, device=0)
+    lv = workloads.uniform_level((4, 4, 4), ((1, 3),) * 3, seed=78)
+    dl = hip.LevelOnDevice(lv)
+    stream = hip.Stream()
+    plan = dl.plan(c, type_=_abi.RHO, precision=_abi.PRECISION_MIXED, mixed_far_d2=4, flags=_abi.PLAN_USE_GRAPH)
+    st = plan.stats()
+    plan.execute(stream.handle)
+    stream.synchronize()
+    L, Lc = dl.L.numpy(), dl.L_c.numpy()
+    plan.close()
+    stream.destroy()
+    dl.free()
+    c.close()
+    assert st["variant"] == variant and st["mixed_far_pairs"] > 0.8 * st["far_pairs"]
+    Lr, Lcr = oracle.level_interactions(lv, theta=0.34, type_=_abi.RHO)
+    tg = lv.targets
+    e = normwise(L[tg], Lr[tg])
+    ok, ft = force_torque_gate(L[tg], Lc[tg], Lr[tg], Lcr[tg], lv.M[tg, :, 0], lv.com[tg], MIXED_FORCE_TOL)
+    print(f"mixed theta 0.34 variant {variant}: FP32 share {st['mixed_far_pairs'] / st['far_pairs']:.3f} "
+          f"normwise {e:.3e} {describe(ft)}", flush=True)
+    assert 0 < e <= MIXED_COMPONENT_TOL and ok
+    assert normwise(Lc[tg], Lcr[tg]) <= TOL
+
+
 def test_level_config3_full_size(ctx):
     """2048 sub-grids of mixed kind (leaf p2p/p2m, non-leaf interior, non-leaf
     next to leaves) in one pass against the oracle."""
