@@ -1373,11 +1373,11 @@ __device__ __forceinline__ double cell_centre(double xb, int i, double dx) {
 // one lane per child cell, 8 consecutive lanes per coarse cell (512 per
 // node).  For RHO the leaf nodes' cell centres are written first; the 8 lanes
 // form the coarse cell's mass and mass-weighted centre by a fixed butterfly,
-// shift their child's multipole to it, and sum the shifted multipoles the
-// same way; lane 0 of the group writes the parent's cell.
-
-
+// shift their child's multipole to it, and sum the shifted multipoles in
+// child order through LDS, each lane summing and storing 2 or 3 components of
+// the parent's cell.
 __global__ __launch_bounds__(256) void m2m_kernel(M2MArgs a) {
+    __shared__ double s_m2m[256 * 21];
     const int gid = blockIdx.x * 256 + threadIdx.x;
     if (gid >= a.nnodes * kCells) return;     // whole 8-lane groups exit together
     const int n = gid >> 9, q = (gid >> 3) & 63, ci = gid & 7;
@@ -1446,11 +1446,23 @@ __global__ __launch_bounds__(256) void m2m_kernel(M2MArgs a) {
         monopole_shift(A[0], dX, B);
     else
         multipole_shift(A, dX, B);
+    // the 8 children's shifted multipoles meet in the wave's LDS rows (21
+    // doubles apart, so the 8 lanes reading one component hit distinct
+    // banks); lane ci of the group then sums components ci, ci + 8 and
+    // ci + 16 over the children in order and stores them, so the group writes
+    // the parent's 160-byte row as consecutive doubles
+    double* row = s_m2m + threadIdx.x * 21;
+#pragma unroll
+    for (int j = 0; j < 20; ++j) row[j] = B[j];
+    lds_wave_sync();
+    const double* grp = s_m2m + (threadIdx.x & ~7) * 21;
     double* Mo = a.pM + ((size_t)pn * kCells + pc) * 20;
 #pragma unroll
-    for (int j = 0; j < 20; ++j) {
-        const double v = sum8(B[j]);
-        if (ci == 0) Mo[j] = v;
+    for (int j = ci; j < 20; j += 8) {
+        double v = grp[j];
+#pragma unroll
+        for (int c = 1; c < 8; ++c) v += grp[c * 21 + j];
+        Mo[j] = v;
     }
 }
 
