@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-interactions/s FP64 vs mixed-prec, % VALU peak, rel. force err vs FP64"
 FP64_VALU_PEAK_TFLOPS = 78.6     # MI355X FP64 vector, dense
+HBM_PEAK_TBS = 8.0               # MI355X HBM3E
 FLOPS_PER_PAIR = {"rho": 407, "drhodt": 273}
 CPU_THREADS_MAX = 16             # host threads one GPU's share of the box gives a process
 
@@ -50,6 +51,13 @@ def parse():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run 3 device-to-device copies of this many MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def staging_bytes(lv, stats):
+    """Algorithmic bytes of one FP64 staging pass over every slot of a
+    uniform level: per cell 184 bytes read (M, com) and a 192-byte record
+    written."""
+    return lv.nslots * 512 * (184 + 192)
 
 
 def kernel_name(variant, rho=True):
@@ -401,9 +409,35 @@ def main():
         "errors_vs_oracle": m4.get("errors_vs_oracle"), "errors_vs_fp64_pass": m4["errors_vs_fp64_pass"]}}
 
     extra = {}
+    def oracle_ref(level):
+        if args.no_check:
+            return None
+        import oracle
+        return oracle.level_interactions(level, nthreads=min(CPU_THREADS_MAX, len(os.sched_getaffinity(0))))
+
+    def mixed_summary(res):
+        # the mixed pass of a level against the oracle (or, without one, the
+        # level's FP64 pass), with its speed against the FP64 pass
+        m = res["mixed"]
+        return {"speedup_vs_fp64": res["fp64"]["ms_per_step"] / m["ms_per_step"],
+                "kernel_speedup_vs_fp64": res["fp64"]["stage_ms"][1] / m["stage_ms"][1],
+                "fp32_far_pair_share": m["stats"]["mixed_far_pairs"] / max(1, m["stats"]["far_pairs"]),
+                "kernel_variant": int(m["stats"]["variant"]),
+                "errors": {k: v for k, v in (m.get("errors_vs_oracle") or m["errors_vs_fp64_pass"]).items()
+                           if k != "normwise_rel_err_per_component"},
+                "errors_against": "oracle" if m.get("errors_vs_oracle") else "fp64 pass"}
+
+    per_config = {}
+    # the single sub-grid of the CPU-runnable configuration, as a one-target level
+    lv1 = workloads.config1()
+    c1 = run_level(ctx, lv1, args, hip, _abi, ref=oracle_ref(lv1), with_drhodt=False)
+    per_config["single-subgrid (C1)"] = mixed_summary(c1)
+    per_config["uniform-level-4096 (C4)"] = mixed_summary(c4)
+
     if not args.no_c2:
         lv2 = workloads.config2()
-        c2 = run_level(ctx, lv2, args, hip, _abi)
+        c2 = run_level(ctx, lv2, args, hip, _abi, ref=oracle_ref(lv2))
+        per_config["uniform-level-512 (C2)"] = mixed_summary(c2)
         s2 = c2["fp64"]["stats"]
         p2 = s2["far_pairs"] + s2["near_pairs"]
         k2 = c2["fp64"]["stage_ms"][1]
@@ -421,7 +455,8 @@ def main():
     kinds = None
     if not args.no_c3:
         lv3 = workloads.config3()
-        c3 = run_level(ctx, lv3, args, hip, _abi, with_mixed=False)
+        c3 = run_level(ctx, lv3, args, hip, _abi, ref=oracle_ref(lv3))
+        per_config["mixed-level-2048 (C3)"] = mixed_summary(c3)
         s3 = c3["fp64"]["stats"]
         ms3 = c3["fp64"]["ms_per_step"]
         p3 = s3["far_pairs"] + s3["near_pairs"] + s3["p2p_pairs"] + s3["p2m_pairs"]
@@ -475,6 +510,12 @@ def main():
             "traffic": traffic,
             "kernel_ms": k_ms,
             "staging_ms": c4["fp64"]["stage_ms"][0],
+            # HBM rates: the kernel's measured bytes, and the staging kernel's
+            # algorithmic bytes (AoS M, com / mon read, records written)
+            "hbm_tbs": (traffic / (k_ms * 1e-3) / 1e12) if traffic else None,
+            "hbm_frac": (traffic / (k_ms * 1e-3) / 1e12 / HBM_PEAK_TBS) if traffic else None,
+            "staging_hbm_tbs": staging_bytes(lv, st) / (c4["fp64"]["stage_ms"][0] * 1e-3) / 1e12,
+            "staging_hbm_frac": staging_bytes(lv, st) / (c4["fp64"]["stage_ms"][0] * 1e-3) / 1e12 / HBM_PEAK_TBS,
             # the dense FP64 FMA rate of this box, re-measured (octofmm_measure_fma_rate)
             "measured_peak": fma_rate["fp64"],
             "frac_of_measured_peak": achieved / fma_rate["fp64"],
@@ -485,6 +526,9 @@ def main():
                              ("max_normwise_rel_err", "force_rel_err", "torque_rel_err")}}
                          if ref is not None else None),
         "mixed_precision": mixed,
+        # FP32 far-pair arithmetic, FP64 accumulation and correction, on every
+        # configuration: speed and errors (the tree's are in tree_solve.mixed)
+        "mixed_precision_per_config": per_config,
         "production_step": c4["production_step"],
         "fma_rate_tflops": fma_rate,
         "uniform_level_512": extra.get("uniform-level-512"),
