@@ -116,6 +116,10 @@ def errors_vs(lv, L, Lc, Lr, Lcr):
     nw = np.abs(a - b).max(axis=0) / np.where(scale > 0, scale, 1.0)
     elem = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
     ft = force_torque_errors(L[tg], Lc[tg], Lr[tg], Lcr[tg], lv.M[tg, :, 0], lv.com[tg])
+    # per-cell acceleration g_i = -(L_i[1..3] + L_c,i): RMS error over RMS value
+    ga = -(L[tg][..., 1:4] + Lc[tg])
+    gb = -(Lr[tg][..., 1:4] + Lcr[tg])
+    g_rms = float(np.sqrt(((ga - gb) ** 2).sum() / max((gb ** 2).sum(), 1e-300)))
     return {
         "max_normwise_rel_err": float(nw.max()),
         "normwise_rel_err_L": float(nw[:20].max()),
@@ -129,6 +133,7 @@ def errors_vs(lv, L, Lc, Lr, Lcr):
         "torque_rel_err_rounding_floor": ft["torque_floor"],
         "force_rel_err_per_cell_sum": ft["force_per_cell_sum"],
         "torque_rel_err_per_cell_sum": ft["torque_per_cell_sum"],
+        "per_cell_force_rms_rel_err": g_rms,
     }
 
 
@@ -423,8 +428,8 @@ def main():
                 "kernel_speedup_vs_fp64": res["fp64"]["stage_ms"][1] / m["stage_ms"][1],
                 "fp32_far_pair_share": m["stats"]["mixed_far_pairs"] / max(1, m["stats"]["far_pairs"]),
                 "kernel_variant": int(m["stats"]["variant"]),
-                "errors": {k: v for k, v in (m.get("errors_vs_oracle") or m["errors_vs_fp64_pass"]).items()
-                           if k != "normwise_rel_err_per_component"},
+                "errors": {k: ([float(f"{x:.3g}") for x in v] if k == "normwise_rel_err_per_component" else v)
+                           for k, v in (m.get("errors_vs_oracle") or m["errors_vs_fp64_pass"]).items()},
                 "errors_against": "oracle" if m.get("errors_vs_oracle") else "fp64 pass"}
 
     per_config = {}

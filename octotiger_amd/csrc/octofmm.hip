@@ -537,6 +537,17 @@ constexpr int kDirWave = 8 * kDirGroup;    // 776 double2 = 12.1 KiB per wave
 typedef __attribute__((address_space(3))) void* lds_ptr;
 // Read-only data seen through the constant address space: uniform loads stay
 // scalar even after the kernel has issued LDS-direct loads.
+// Forms the 8 values in scalar registers at this point, so that the scalar
+// loads they depend on are in flight together (the constraint exists only in
+// the device pass).
+__device__ __forceinline__ void form_sgprs(const unsigned (&v)[8]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" ::"s"(v[0]), "s"(v[1]), "s"(v[2]), "s"(v[3]), "s"(v[4]), "s"(v[5]), "s"(v[6]), "s"(v[7]));
+#else
+    (void)v;
+#endif
+}
+
 template <typename T>
 __device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* p) {
     return (const __attribute__((address_space(4))) T*)p;
@@ -560,7 +571,14 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     int w, double2* __restrict__ s_stage, int sl = 0) {
     // this wave's image (wave-uniform address)
     double2* stage = s_stage + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * kDirWave;
-    const int l = threadIdx.x & 63, c = l >> 3;
+    // an opaque copy of the lane index: lane values formed from it are formed
+    // where they are used instead of being held across the pair loop, where
+    // the RHO kernels run at their register limit
+    auto lane = []() -> int {
+        int t;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((int)threadIdx.x));
+        return t & 63;
+    };
     const auto* tab = as_const(a.tab);
     const int wu = __builtin_amdgcn_readfirstlane(w);
     const double* trec = a.rec + ((size_t)slot * kCells + lane_geom(wu).rec) * kRecDoubles;
@@ -575,73 +593,133 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
     const int nd = tab->ndelta;
     const int bx = ((wu >> 2) & 1) * 2, by = ((wu >> 1) & 1) * 2, bz = (wu & 1) * 2;
     const auto* nbr_row = as_const(a.nbr + (size_t)__builtin_amdgcn_readfirstlane(slot) * 27);
-    // partner coarse cell of group gi (coarse cell gi of the wave's block) at
-    // offset a; empty directions are marked in bit gi of `empty`
-    auto group_base = [&](const double* src, int gi, int ax, int ay, int az, unsigned& empty) -> const double2* {
+    // record index (slot * 512 + first child) of the partner coarse cell of
+    // group gi (coarse cell gi of the wave's block) at offset a; empty
+    // directions are marked in bit gi of `empty`.  32-bit indices keep the 8
+    // bases of the next image in 8 scalar registers across the pair loop.
+    auto group_rec = [&](int gi, int ax, int ay, int az, unsigned& empty) -> unsigned {
         const int px = bx + ((gi >> 2) & 1) + ax, py = by + ((gi >> 1) & 1) + ay, pz = bz + (gi & 1) + az;
         const int raw = (px < 0 ? 0 : (px < 4 ? 1 : 2)) + 3 * (py < 0 ? 0 : (py < 4 ? 1 : 2)) +
             9 * (pz < 0 ? 0 : (pz < 4 ? 1 : 2));
         const int ns = nbr_row[raw];
         const int cb = ((px & 3) * 16 + (py & 3) * 4 + (pz & 3)) * 8;
         if (ns < 0) empty |= 1u << gi;
-        return reinterpret_cast<const double2*>(src + ((size_t)(ns < 0 ? 0 : ns) * kCells + cb) * kRecDoubles);
+        return (unsigned)(ns < 0 ? 0 : ns) * (unsigned)kCells + (unsigned)cb;
     };
     // Group c (8 records = 96 chunks) is loaded by one full instruction
     // (chunks 0..63) and one on the lower 32 lanes (chunks 64..95): both are a
     // uniform base plus the lane's 16-byte offset, and the LDS destination is
     // uniform too, so a refill costs one address add per load instruction.
-    const unsigned loff = (unsigned)l * 16u;
     // The 8 partner bases of an offset are formed ahead of its refill, their
     // neighbour-slot loads in flight together (one wait, taken while the
     // current image is still arriving, instead of one wait per group).
-    const char* nbase[8];
-    unsigned empty_next = 0;   // groups of the offset bases() last formed whose partner slot is empty
-    auto bases = [&](int di, const double* src) {
+    unsigned nrec[8];
+    const double* src_next = a.rec;   // the records the entry bases() last formed is staged from
+    unsigned empty_next = 0;    // groups of that entry whose partner (a) slot is empty
+    unsigned emptyb_next = 0;   // the same for offset b of a split entry (formed by issue())
+    int entry_next = 0;         // the schedule entry bases() last formed
+    auto bases = [&](int q, const double* src) {
+        const int e = tab->sched[q];
+        const int di = e & 0xff;
         const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
         unsigned empty = 0;
 #pragma unroll
-        for (int gi = 0; gi < 8; ++gi)
-            nbase[gi] = reinterpret_cast<const char*>(group_base(src, gi, ax, ay, az, empty));
-        asm volatile("" ::"s"(nbase[0]), "s"(nbase[1]), "s"(nbase[2]), "s"(nbase[3]), "s"(nbase[4]),
-                     "s"(nbase[5]), "s"(nbase[6]), "s"(nbase[7]));
+        for (int gi = 0; gi < 8; ++gi) nrec[gi] = group_rec(gi, ax, ay, az, empty);
+        form_sgprs(nrec);
         empty_next = empty;
+        entry_next = e;
+        src_next = src;
     };
     // The full-wave loads of all groups come first and the half-wave loads
     // follow in one branch of their own: with a full and a half load of one
     // group on the two sides of a branch, the compiler may merge them into one
     // load whose LDS base differs by lane and is then taken from lane 0, which
     // sends the other lanes' chunks to the wrong place.
+    //
+    // A split entry (kinds 1 and 2) stages 4 records of offset a's partner
+    // and 4 of offset b's per group, in two loads on lanes 0..47: lane l
+    // moves chunk l % 12 of the half's (l / 12)-th child with the given bit
+    // on the entry's axis.  Offset b is -a; its partner bases are formed once
+    // a's loads are issued (split entries
+    // always stage the FP64 records).  The LDS bases of all loads, and the lane
+    // offsets of the split loads, come from opaque copies, so that they are
+    // formed here and not held across the pair loop, where the kernel runs at
+    // its register limit.
     auto issue = [&]() {
-#pragma unroll
-        for (int gi = 0; gi < 8; ++gi)
-            __builtin_amdgcn_global_load_lds(nbase[gi] + loff, (lds_ptr)(stage + gi * kDirGroup), 16, 0, 0);
-        if (l < 32) {
+        const int kind = (entry_next >> 16) & 0xff;
+        double2* st = stage;
+        asm volatile("" : "+s"(st));
+        const int lo = lane();
+        if (kind == 0) {
+            const char* rb = reinterpret_cast<const char*>(src_next);
+            const unsigned loff = (unsigned)lo * 16u;
 #pragma unroll
             for (int gi = 0; gi < 8; ++gi)
-                __builtin_amdgcn_global_load_lds(nbase[gi] + 1024 + loff, (lds_ptr)(stage + gi * kDirGroup + 64), 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(rb + (size_t)nrec[gi] * kRecBytes + loff, (lds_ptr)(st + gi * kDirGroup),
+                                                 16, 0, 0);
+            if (lo < 32) {
+#pragma unroll
+                for (int gi = 0; gi < 8; ++gi)
+                    __builtin_amdgcn_global_load_lds(rb + (size_t)nrec[gi] * kRecBytes + 1024 + loff,
+                                                     (lds_ptr)(st + gi * kDirGroup + 64), 16, 0, 0);
+            }
+        } else {
+            const char* rb = reinterpret_cast<const char*>(a.rec);
+            const int axis = entry_next >> 24;
+            const int r = lo / 12, part = lo - 12 * r;
+            const int low = (1 << axis) - 1;
+            const int jr = ((r & ~low) << 1) | (r & low);   // r-th child with a 0 bit on the axis
+            const int va = kind == 1 ? 1 : 0;               // the axis bit of a's staged children
+            const unsigned offa = (unsigned)(((jr | (va << axis)) * 12 + part) * 16);
+            const unsigned offb = (unsigned)(((jr | ((1 - va) << axis)) * 12 + part) * 16);
+            if (lo < 48) {
+#pragma unroll
+                for (int gi = 0; gi < 8; ++gi)
+                    __builtin_amdgcn_global_load_lds(rb + (size_t)nrec[gi] * kRecBytes + offa, (lds_ptr)(st + gi * kDirGroup),
+                                                     16, 0, 0);
+            }
+            const int di = entry_next & 0xff;
+            const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
+            unsigned brec[8];
+            unsigned emptyb = 0;
+#pragma unroll
+            for (int gi = 0; gi < 8; ++gi) brec[gi] = group_rec(gi, -ax, -ay, -az, emptyb);
+            form_sgprs(brec);
+            emptyb_next = emptyb;
+            if (lo < 48) {
+#pragma unroll
+                for (int gi = 0; gi < 8; ++gi)
+                    __builtin_amdgcn_global_load_lds(rb + (size_t)brec[gi] * kRecBytes + offb,
+                                                     (lds_ptr)(st + gi * kDirGroup + 48), 16, 0, 0);
+            }
         }
     };
-    // the lane's far children at offset di: none when its partner coarse
-    // cell (group c) lies in an empty direction
-    auto far_bits = [&](int di, unsigned empty) -> unsigned {
-        const unsigned fb = (unsigned)(tab->farbits[di] >> (8 * (l & 7))) & 0xffu;
-        return ((empty >> c) & 1u) ? 0u : fb;
-    };
-    const double2* group = stage + c * kDirGroup;
-    // only the offsets with a far pair are staged (delta 0 is all near / self)
-    const int nfd = tab->nfar_delta;
+    // only the offsets with a far pair are staged (delta 0 is all near / self),
+    // in the order of the far schedule
+    const int nfd = tab->nsched;
     const int q0 = SPLIT == 1 ? 0 : __builtin_amdgcn_readfirstlane(sl);
     const auto* mixoff = as_const(a.mixoff);
-    // the records offset q is staged from
+    // the records entry q is staged from
     auto source = [&](int q) -> const double* { return (MIXED && mixoff[q]) ? a.recm : a.rec; };
     if (q0 < nfd) {
-        bases(tab->far_delta[q0], source(q0));
+        bases(q0, source(q0));
         issue();
     }
     for (int q = q0; q < nfd; q += SPLIT) {
-        const unsigned fb = far_bits(tab->far_delta[q], empty_next);
+        // the lane's records of the image: those of its class from each
+        // partner, unless that partner's slot is empty; kind 2 classes on the
+        // far side of the axis read records 4..7 as their records 0..3
+        const int lq = lane();
+        const unsigned k = (unsigned)lq & 7u, c = (unsigned)lq >> 3;
+        const uint64_t ma = tab->sched_fara[q], mb = tab->sched_farb[q];
+        const unsigned wa = k < 4 ? (unsigned)ma : (unsigned)(ma >> 32);
+        const unsigned wb = k < 4 ? (unsigned)mb : (unsigned)(mb >> 32);
+        const unsigned fa = ((empty_next >> c) & 1u) ? 0u : wa;
+        const unsigned fbb = ((emptyb_next >> c) & 1u) ? 0u : wb;
+        const unsigned fb = __builtin_amdgcn_ubfe(fa | fbb, 8 * (k & 3u), 8);
+        const double2* recs = stage + c * kDirGroup + ((tab->sched_half[q] >> k) & 1u) * 48;
         const bool f32 = MIXED && mixoff[q];   // uniform
-        if (q + SPLIT < nfd) bases(tab->far_delta[q + SPLIT], source(q + SPLIT));
+        if (q + SPLIT < nfd) bases(q + SPLIT, source(q + SPLIT));
         __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): this wave's image is in LDS
         __builtin_amdgcn_wave_barrier();
         if (MIXED && f32) {
@@ -650,7 +728,7 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             for (int i = 0; i < 20; ++i) Lf[i] = 0.0f;
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                if (fb & (1u << j)) m2l_far_m32<RHO>(group + j * 12, X0, X1, X2, inv_mc0, mc, Lf, C);
+                if (fb & (1u << j)) m2l_far_m32<RHO>(recs + j * 12, X0, X1, X2, inv_mc0, mc, Lf, C);
             // the offset's FP32 partial sums join the FP64 expansion
 #pragma unroll
             for (int i = 0; i < 20; ++i) L[i] += (double)Lf[i];
@@ -660,7 +738,7 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
                 if (fb & (1u << j)) {
                     Rec cur;
 #pragma unroll
-                    for (int i = 0; i < 12; ++i) cur.v[i] = group[j * 12 + i];
+                    for (int i = 0; i < 12; ++i) cur.v[i] = recs[j * 12 + i];
                     m2l_far_rec<RHO>(cur, X0, X1, X2, inv_mc0, mc, L, C);
                 }
             }
@@ -1617,7 +1695,7 @@ struct octofmm_plan {
     std::vector<int2> h_items;     // the work items (host copy, for the tuning key)
     uint64_t* d_mixbits = nullptr;
     // mixed precision, lane mappings 3 and 7: the mixed-precision records and
-    // the far offsets evaluated in FP32 as a whole (far_delta order)
+    // the far-schedule entries evaluated in FP32 as a whole (Tables::sched order)
     double* d_recm = nullptr;
     uint8_t* d_mixoff = nullptr;
     uint32_t* d_queue = nullptr;   // per-XCD work counters of the persistent variants
@@ -2653,11 +2731,15 @@ static octofmm_status level_plan_build(octofmm_context* c, const octofmm_level* 
             hipMemcpy(p->d_mixbits, mb.data(), sizeof(uint64_t) * mb.size(), hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(OCTOFMM_ERR_ALLOCATION, "mixed-precision table upload failed");
         // lane mappings 3 and 7 choose the precision per far coarse offset:
-        // FP32 where every far pair of the offset is an FP32 pair above
-        std::vector<uint8_t> mo(T.nfar_delta > 0 ? T.nfar_delta : 1, 0);
+        // FP32 where every far pair of the offset is an FP32 pair above.  The
+        // split entries of opposite face offsets stage FP64 records only, so
+        // their offsets stay FP64 (at mixed_far_d2 >= 4 they hold |d| = 2
+        // pairs and would anyway)
+        std::vector<uint8_t> mo(T.nsched > 0 ? T.nsched : 1, 0);
         std::vector<char> whole(T.ndelta > 0 ? T.ndelta : 1, 0);
-        for (int q = 0; q < T.nfar_delta; ++q) {
-            const int di = T.far_delta[q];
+        for (int q = 0; q < T.nsched; ++q) {
+            if ((T.sched[q] >> 16) & 0xff) continue;
+            const int di = T.sched[q] & 0xff;
             whole[di] = T.farbits[di] != 0 && mb[di] == T.farbits[di];
             mo[q] = whole[di] ? 1 : 0;
         }

@@ -12,6 +12,7 @@ namespace octofmm {
 constexpr int kInx = 8;
 constexpr int kCells = 512;
 constexpr int kRecDoubles = 24;       // partner record: m0, M1..M19, Y0..Y2, mon
+constexpr int kRecBytes = kRecDoubles * 8;
 constexpr int kRecY = 20;
 constexpr int kRecMon = 23;
 constexpr int kMaxDelta = 125;        // coarse offsets in [-2, 2]^3 (theta >= 0.34)
@@ -40,6 +41,31 @@ struct Tables {
     uint64_t nearbits[kMaxDelta];     // byte k: bit j set if (delta, j, k) is near (self excluded)
     int32_t nfar_delta;               // coarse offsets with a far pair for some child class,
     int32_t far_delta[kMaxDelta];     // in delta order (delta 0 has none: it is all near / self)
+    // Far schedule of the direct-staging kernels: one staged image of 8
+    // partner records per coarse cell for each entry, packed as
+    // a | b << 8 | kind << 16 | axis << 24 (a, b delta indices; axis = the
+    // child-index bit of x, y or z: 2, 1, 0).
+    //   kind 0: the 8 children of offset a, in child order.
+    //   kinds 1 and 2 take a pair of opposite face offsets a = +e, b = -e
+    //   when its far bits are those of theta = 0.5: far(a, k, j) iff
+    //   j_axis = 1 or k_axis = 0, and far(b, k, j) iff j_axis = 0 or
+    //   k_axis = 1.  Walked as two plain images, a class would idle on 4 of
+    //   the 8 children of one of them.  Instead,
+    //   kind 1 stages a's children with j_axis = 1 and b's with j_axis = 0
+    //   (records 0..3 and 4..7, each in child order), far for every class;
+    //   kind 2 stages a's children with j_axis = 0 and b's with j_axis = 1,
+    //   and class k reads only the half of its own side, k_axis = 0 the
+    //   first, k_axis = 1 the second: 12 steps for the pair instead of 16,
+    //   each far pair once.
+    // Per entry and class k (byte k): the records class k reads, as bits of
+    // the image's records 0..7 taken from offset a's partner (sched_fara)
+    // and from b's (sched_farb), and bit k of sched_half set where class k
+    // reads records 4..7 as its records 0..3 (kind 2, k_axis = 1).
+    int32_t nsched;
+    int32_t sched[kMaxDelta];
+    uint64_t sched_fara[kMaxDelta];
+    uint64_t sched_farb[kMaxDelta];
+    uint32_t sched_half[kMaxDelta];
     double four[kStencilBox][4];      // p2p constants (-1/r, x/r^3, y/r^3, z/r^3) by fine offset
     uint8_t root_excluded[kStencilBox];   // root kernel: near offsets and self
     uint8_t phase[kStencilBox];       // 1 far, 0 near (inside the stencil), 1 elsewhere

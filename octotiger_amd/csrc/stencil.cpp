@@ -33,6 +33,74 @@ inline int box_index(int x, int y, int z) {
     return (x + 5) * kStencilInx * kStencilInx + (y + 5) * kStencilInx + (z + 5);
 }
 
+int find_delta(const Tables& t, int x, int y, int z) {
+    for (int i = 0; i < t.ndelta; ++i)
+        if (t.delta[i][0] == x && t.delta[i][1] == y && t.delta[i][2] == z) return i;
+    return -1;
+}
+
+// The far schedule (Tables::sched): the far offsets in delta order, each
+// pair of opposite face offsets whose far bits have the split form replaced
+// by its two split images.  The three kind-2 entries (4 steps each) go to
+// positions 0, 1 and 4, which fall to the waves with the most entries when
+// 2, 4 or 8 waves take every 2nd, 4th or 8th entry of the 26 at theta = 0.5.
+void build_far_schedule(Tables& t) {
+    std::vector<int> paired(kMaxDelta, 0);
+    std::vector<int32_t> split1, split2;
+    for (int axis = 2; axis >= 0; --axis) {
+        int e[3] = {0, 0, 0};
+        e[2 - axis] = 1;   // child-index bit 2 is x, 1 is y, 0 is z
+        const int a = find_delta(t, e[0], e[1], e[2]), b = find_delta(t, -e[0], -e[1], -e[2]);
+        if (a < 0 || b < 0) continue;
+        bool split = true;
+        for (int k = 0; k < 8; ++k)
+            for (int j = 0; j < 8; ++j) {
+                const int jb = (j >> axis) & 1, kb = (k >> axis) & 1;
+                const bool fa = (t.farbits[a] >> (8 * k + j)) & 1, fb = (t.farbits[b] >> (8 * k + j)) & 1;
+                split = split && fa == (jb == 1 || kb == 0) && fb == (jb == 0 || kb == 1);
+            }
+        if (!split) continue;
+        paired[a] = paired[b] = 1;
+        split1.push_back(a | b << 8 | 1 << 16 | axis << 24);
+        split2.push_back(a | b << 8 | 2 << 16 | axis << 24);
+    }
+    std::vector<int32_t> plain;
+    for (int q = 0; q < t.nfar_delta; ++q)
+        if (!paired[t.far_delta[q]]) plain.push_back(t.far_delta[q]);
+    for (int32_t s : split1) plain.push_back(s);
+    const int n = (int)plain.size() + (int)split2.size();
+    const int pos2[3] = {0, 1, 4};
+    size_t i1 = 0, i2 = 0;
+    for (int q = 0; q < n; ++q) {
+        const bool take2 = i2 < split2.size() && q == pos2[i2];
+        const int32_t e = take2 ? split2[i2++] : plain[i1++];
+        const int kind = (e >> 16) & 0xff, axis = e >> 24;
+        uint64_t fa = 0, fb = 0;
+        uint32_t half = 0;
+        for (int k = 0; k < 8; ++k) {
+            const int kb = (k >> axis) & 1;
+            uint64_t ma = 0, mb = 0;
+            if (kind == 0) {
+                ma = (t.farbits[e & 0xff] >> (8 * k)) & 0xff;
+            } else if (kind == 1) {
+                ma = 0x0f;
+                mb = 0xf0;
+            } else {
+                ma = kb ? 0 : 0x0f;
+                mb = kb ? 0x0f : 0;
+                half |= (uint32_t)kb << k;
+            }
+            fa |= ma << (8 * k);
+            fb |= mb << (8 * k);
+        }
+        t.sched[q] = e;
+        t.sched_fara[q] = fa;
+        t.sched_farb[q] = fb;
+        t.sched_half[q] = half;
+    }
+    t.nsched = n;
+}
+
 }   // namespace
 
 bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
@@ -134,6 +202,7 @@ bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
     t.nfar_delta = 0;
     for (int i = 0; i < n; ++i)
         if (t.farbits[i]) t.far_delta[t.nfar_delta++] = i;
+    build_far_schedule(t);
     int nf = 0, nn = 0;
     for (int i = 0; i < n; ++i) {
         nf += __builtin_popcountll(t.farbits[i] & 0xffull);
