@@ -39,11 +39,15 @@ int find_delta(const Tables& t, int x, int y, int z) {
     return -1;
 }
 
-// The far schedule (Tables::sched): the far offsets in delta order, each
-// pair of opposite face offsets whose far bits have the split form replaced
-// by its two split images.  The three kind-2 entries (4 steps each) go to
-// positions 0, 1 and 4, which fall to the waves with the most entries when
-// 2, 4 or 8 waves take every 2nd, 4th or 8th entry of the 26 at theta = 0.5.
+// The far schedule (Tables::sched): the split images of each pair of
+// opposite face offsets whose far bits have the split form, then the other
+// far offsets in delta order.  The split images come first as
+// [2x, 2y, 1x, 1y, 2z, 1z] (kind, axis): the three kind-2 entries (4 steps
+// each) sit at positions 0, 1 and 4, which fall to the waves with the most
+// entries when 2, 4 or 8 waves take every 2nd, 4th or 8th entry of the 26 at
+// theta = 0.5, and the two images of a pair, which read the same partner
+// records, are staged together or one after the other, so the second read
+// of those records finds them in the L2 cache.
 void build_far_schedule(Tables& t) {
     std::vector<int> paired(kMaxDelta, 0);
     std::vector<int32_t> split1, split2;
@@ -64,16 +68,21 @@ void build_far_schedule(Tables& t) {
         split1.push_back(a | b << 8 | 1 << 16 | axis << 24);
         split2.push_back(a | b << 8 | 2 << 16 | axis << 24);
     }
-    std::vector<int32_t> plain;
+    std::vector<int32_t> order;
+    const size_t np = split1.size();
+    if (np > 0) order.push_back(split2[0]);
+    if (np > 1) order.push_back(split2[1]);
+    if (np > 0) order.push_back(split1[0]);
+    if (np > 1) order.push_back(split1[1]);
+    if (np > 2) {
+        order.push_back(split2[2]);
+        order.push_back(split1[2]);
+    }
     for (int q = 0; q < t.nfar_delta; ++q)
-        if (!paired[t.far_delta[q]]) plain.push_back(t.far_delta[q]);
-    for (int32_t s : split1) plain.push_back(s);
-    const int n = (int)plain.size() + (int)split2.size();
-    const int pos2[3] = {0, 1, 4};
-    size_t i1 = 0, i2 = 0;
+        if (!paired[t.far_delta[q]]) order.push_back(t.far_delta[q]);
+    const int n = (int)order.size();
     for (int q = 0; q < n; ++q) {
-        const bool take2 = i2 < split2.size() && q == pos2[i2];
-        const int32_t e = take2 ? split2[i2++] : plain[i1++];
+        const int32_t e = order[q];
         const int kind = (e >> 16) & 0xff, axis = e >> 24;
         uint64_t fa = 0, fb = 0;
         uint32_t half = 0;
