@@ -300,34 +300,47 @@ __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int ce
 // LDS, no longer in use.
 __device__ __forceinline__ void leaf_rows_coalesced(const LevelArgs& a, int slot, int cell, const double* Lf,
     const double* Cf, const double* g4, double* __restrict__ buf) {
+    // Each staged row, L_c triple and g quadruple is shifted by one more
+    // element every 16 cells (c >> 4): the lanes of a wave hold cells 2 apart
+    // in z, 16 in y and 128 in x, and without the shift their rows fall on a
+    // quarter of the LDS banks (8-way conflicts on the row writes, 4-way on
+    // L_c, 8-way on g); with it each half-wave spreads over all of them.
     const int t = threadIdx.x;
     double2* b2 = reinterpret_cast<double2*>(buf);
     for (int half = 0; half < (a.leaf_rows ? 2 : 0); ++half) {   // uniform
         __syncthreads();
         if ((cell >> 8) == half) {
-            double2* row = b2 + (cell & 255) * 10;
+            const int c = cell & 255;
+            double2* row = b2 + c * 10 + (c >> 4);
 #pragma unroll
             for (int j = 0; j < 10; ++j) row[j] = make_double2(Lf[2 * j], Lf[2 * j + 1]);
         }
         __syncthreads();
         double2* dst = reinterpret_cast<double2*>(a.L + ((size_t)slot * kCells + half * 256) * 20);
 #pragma unroll
-        for (int r = 0; r < 5; ++r) dst[t + 512 * r] = b2[t + 512 * r];   // 256 rows x 10 chunks
+        for (int r = 0; r < 5; ++r) {   // 256 rows x 10 chunks
+            const int ch = t + 512 * r;
+            dst[ch] = b2[ch + (ch / 10 >> 4)];
+        }
     }
     __syncthreads();
-    buf[cell * 3 + 0] = Cf[0];
-    buf[cell * 3 + 1] = Cf[1];
-    buf[cell * 3 + 2] = Cf[2];
-    double2* bg = b2 + 768;
-    bg[cell * 2] = make_double2(g4[0], g4[1]);
-    bg[cell * 2 + 1] = make_double2(g4[2], g4[3]);
+    buf[cell * 3 + (cell >> 4) + 0] = Cf[0];
+    buf[cell * 3 + (cell >> 4) + 1] = Cf[1];
+    buf[cell * 3 + (cell >> 4) + 2] = Cf[2];
+    double2* bg = b2 + 784;   // after the 1536 + 32 doubles of L_c
+    bg[cell * 2 + (cell >> 4)] = make_double2(g4[0], g4[1]);
+    bg[cell * 2 + (cell >> 4) + 1] = make_double2(g4[2], g4[3]);
     __syncthreads();
     double2* lc = reinterpret_cast<double2*>(a.Lc + (size_t)slot * kCells * 3);
     double2* gg = reinterpret_cast<double2*>(a.g + (size_t)slot * kCells * 4);
-    lc[t] = b2[t];                                    // 768 chunks of L_c
-    if (t < 256) lc[512 + t] = b2[512 + t];
-    gg[t] = bg[t];                                    // 1024 chunks of g
-    gg[512 + t] = bg[512 + t];
+    auto lc_pair = [&](int ch) {   // doubles 2 ch and 2 ch + 1 of the L_c block
+        const int d0 = 2 * ch, d1 = d0 + 1;
+        return make_double2(buf[d0 + (d0 / 3 >> 4)], buf[d1 + (d1 / 3 >> 4)]);
+    };
+    lc[t] = lc_pair(t);                               // 768 chunks of L_c
+    if (t < 256) lc[512 + t] = lc_pair(512 + t);
+    gg[t] = bg[t + (t >> 5)];                         // 1024 chunks of g (cell = chunk / 2)
+    gg[512 + t] = bg[512 + t + ((512 + t) >> 5)];
 }
 
 template <bool RHO, bool MIXED>
@@ -807,6 +820,19 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
 // stencil, far and near alike (monopole_kernel_templates.hpp:69-239): those
 // sums come from p2m_kernel, launched before this kernel on the same stream.
 
+// LDS layout of the 14^3 monopole window of the theta = 0.5 leaf kernel: 8
+// parity planes (x & 1, y & 1, z & 1) of 7^3 cells each, row stride 12 and
+// plane stride 80 doubles.  At each partner offset a wave's 64 lanes read
+// cells 2C + const over the 4^3 coarse cells C, i.e. consecutive cells of
+// one parity plane: with strides 1, 12 and 80 (mod 32: 1, 12, 16) the 32
+// lanes of each half-wave hit 32 distinct 8-byte bank pairs.  In the plain
+// x-slowest layout the lanes' stride of 2 cells put 4 of them on every bank
+// pair in use.
+constexpr int kWinSY = 12, kWinSX = 80, kWinPlane = 7 * kWinSX;
+__device__ __forceinline__ int cube_win(int wx, int wy, int wz) {
+    return (((wx & 1) << 2) | ((wy & 1) << 1) | (wz & 1)) * kWinPlane + (wx >> 1) * kWinSX + (wy >> 1) * kWinSY + (wz >> 1);
+}
+
 template <bool RHO, bool CUBE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void leaf_kernel(LevelArgs a) {
     extern __shared__ double s_win[];
@@ -832,7 +858,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
         double m = 0.0;
         if (ns >= 0 && s_kind[raw] == OCTOFMM_SLOT_LEAF)
             m = a.mon[(size_t)ns * kCells + ((x + kInx) & 7) * 64 + ((y + kInx) & 7) * 8 + ((z + kInx) & 7)];
-        s_win[i] = m;
+        if constexpr (CUBE) {
+            const int wx = x + h, wy = y + h, wz = z + h;
+            s_win[cube_win(wx, wy, wz)] = m;
+        } else {
+            s_win[i] = m;
+        }
     };
     if constexpr (CUBE) {
         constexpr int kWin = 14 * 14 * 14;
@@ -856,7 +887,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
         // window offsets as LDS immediates and the constants of offset
         // d = p - k at fixed offsets from one class base:
         // box(d) = box(p) + 3 * 133 - (121 kx + 11 ky + kz) for p in [0, 6)^3
-        const double* wc = s_win + ((2 * Cx + 1) * 14 + (2 * Cy + 1)) * 14 + (2 * Cz + 1);
+        // window cell 2C + 1 + p lies in parity plane (1 + p) & 1 at index
+        // C + ((1 + p) >> 1): the lane's base plus a compile-time offset
+        const double* wc = s_win + Cx * kWinSX + Cy * kWinSY + Cz;
         const double(*__restrict__ fb)[4] = tab->four + (3 * 133 - (121 * kx + 11 * ky + kz));
 #pragma unroll
         for (int px = 0; px < 6; ++px)
@@ -864,7 +897,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void l
             for (int py = 0; py < 6; ++py)
 #pragma unroll
                 for (int pz = 0; pz < 6; ++pz) {
-                    const double m = wc[(px * 14 + py) * 14 + pz];
+                    const double m = wc[cube_win(1 + px, 1 + py, 1 + pz)];
                     const double* f = fb[px * 121 + py * 11 + pz];
                     p0 = fma(f[0], m, p0);
                     p1 = fma(f[1], m, p1);
@@ -1085,8 +1118,10 @@ template <bool RHO>
 void launch_leaf(int nitems, hipStream_t st, const LevelArgs& la, const Tables& tab) {
     const int W = kInx + 2 * tab.p2p_halo;
     const bool cube = tab.p2p_cube != 0;
-    // window, and the staged L rows of half a sub-grid (256 x 20)
-    const size_t lds = sizeof(double) * std::max<size_t>((size_t)W * W * W, (size_t)256 * 20);
+    // window (parity planes on the cube path), and the staged L rows of half
+    // a sub-grid (256 x 20, shifted by one chunk every 16 rows)
+    const size_t win = cube ? (size_t)8 * kWinPlane : (size_t)W * W * W;
+    const size_t lds = sizeof(double) * std::max<size_t>(win, (size_t)256 * 20 + 32);
     if (cube)
         hipLaunchKernelGGL((leaf_kernel<RHO, true>), dim3(nitems), dim3(512), lds, st, la);
     else
