@@ -298,6 +298,14 @@ __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int ce
 // LDS), then L_c and g together (28 KiB); without a.leaf_rows only L_c and
 // g.  Every thread of the workgroup calls it; buf is the kernel's dynamic
 // LDS, no longer in use.
+// A store of a leaf's final rows, which nothing in the solve reads again:
+// non-temporal, so the lines stream out without displacing the records and
+// expansions the concurrent level kernels read through L2.
+__device__ __forceinline__ void store_stream(double2* p, double2 v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
 __device__ __forceinline__ void leaf_rows_coalesced(const LevelArgs& a, int slot, int cell, const double* Lf,
     const double* Cf, const double* g4, double* __restrict__ buf) {
     // Each staged row, L_c triple and g quadruple is shifted by one more
@@ -320,7 +328,7 @@ __device__ __forceinline__ void leaf_rows_coalesced(const LevelArgs& a, int slot
 #pragma unroll
         for (int r = 0; r < 5; ++r) {   // 256 rows x 10 chunks
             const int ch = t + 512 * r;
-            dst[ch] = b2[ch + (ch / 10 >> 4)];
+            store_stream(dst + ch, b2[ch + (ch / 10 >> 4)]);
         }
     }
     __syncthreads();
@@ -337,10 +345,10 @@ __device__ __forceinline__ void leaf_rows_coalesced(const LevelArgs& a, int slot
         const int d0 = 2 * ch, d1 = d0 + 1;
         return make_double2(buf[d0 + (d0 / 3 >> 4)], buf[d1 + (d1 / 3 >> 4)]);
     };
-    lc[t] = lc_pair(t);                               // 768 chunks of L_c
-    if (t < 256) lc[512 + t] = lc_pair(512 + t);
-    gg[t] = bg[t + (t >> 5)];                         // 1024 chunks of g (cell = chunk / 2)
-    gg[512 + t] = bg[512 + t + ((512 + t) >> 5)];
+    store_stream(lc + t, lc_pair(t));                 // 768 chunks of L_c
+    if (t < 256) store_stream(lc + 512 + t, lc_pair(512 + t));
+    store_stream(gg + t, bg[t + (t >> 5)]);           // 1024 chunks of g (cell = chunk / 2)
+    store_stream(gg + 512 + t, bg[512 + t + ((512 + t) >> 5)]);
 }
 
 template <bool RHO, bool MIXED>
