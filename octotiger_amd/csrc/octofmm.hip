@@ -50,6 +50,15 @@ __device__ __forceinline__ int xcd_item(int b, int n) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// A store of data nothing in the same pass or solve reads through L2 again
+// (staged records, which the level kernels read once per image from HBM, and
+// a leaf's final rows): non-temporal, so the lines stream out without
+// displacing what the concurrent kernels reuse.
+__device__ __forceinline__ void store_stream(double2* p, double2 v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
 // One block per 128 cells of a slot (two x-planes).  Those cells' records are
 // the 128 consecutive records of the block's coarse x-layer, so the block
 // loads the cells' M and centre-of-mass rows (contiguous) into LDS in 16-byte
@@ -111,7 +120,7 @@ __global__ __launch_bounds__(256) void stage_records(StageArgs a) {
                 v.y = m;
             }
         }
-        dst[ch] = v;
+        store_stream(dst + ch, v);
         if (a.recm) {
             // the mixed-precision record of the same cell: m0..m19 rounded to
             // FP32 (4 per chunk), (Y0, Y1), (Y2, m0), the octupole in FP64
@@ -146,7 +155,7 @@ __global__ __launch_bounds__(256) void stage_records(StageArgs a) {
                     w.y = m;
                 }
             }
-            reinterpret_cast<double2*>(a.recm + row0 * kRecDoubles)[ch] = w;
+            store_stream(reinterpret_cast<double2*>(a.recm + row0 * kRecDoubles) + ch, w);
         }
     }
 }
@@ -298,14 +307,6 @@ __device__ __forceinline__ void finish_cell(const LevelArgs& a, int slot, int ce
 // LDS), then L_c and g together (28 KiB); without a.leaf_rows only L_c and
 // g.  Every thread of the workgroup calls it; buf is the kernel's dynamic
 // LDS, no longer in use.
-// A store of a leaf's final rows, which nothing in the solve reads again:
-// non-temporal, so the lines stream out without displacing the records and
-// expansions the concurrent level kernels read through L2.
-__device__ __forceinline__ void store_stream(double2* p, double2 v) {
-    __builtin_nontemporal_store(v.x, &p->x);
-    __builtin_nontemporal_store(v.y, &p->y);
-}
-
 __device__ __forceinline__ void leaf_rows_coalesced(const LevelArgs& a, int slot, int cell, const double* Lf,
     const double* Cf, const double* g4, double* __restrict__ buf) {
     // Each staged row, L_c triple and g quadruple is shifted by one more
