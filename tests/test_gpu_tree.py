@@ -118,8 +118,12 @@ def test_tree_replays_clear_and_repeat(ctx):
     dev.free()
 
 
-def test_tree_without_graph_matches_graph(ctx):
-    t = workloads.small_tree()
+@pytest.mark.parametrize("name", ["small", "c5"])
+def test_tree_without_graph_matches_graph(ctx, name):
+    """Eager launches on the plan's streams give the graph's bits, on the
+    small tree and on C5 (four streams, p2m and leaf kernels of every level)."""
+    t = workloads.small_tree() if name == "small" else workloads.config5()
+    print(f"tree {t.name}", flush=True)
     _, d1, a = run(ctx, t, flags=_abi.PLAN_USE_GRAPH)
     _, d2, b = run(ctx, t, flags=0)
     for x, y in zip(a, b):
