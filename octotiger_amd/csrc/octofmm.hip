@@ -2530,12 +2530,15 @@ static octofmm_status plan_autotune_run(octofmm_plan* p) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // Within 1% of the fastest (the spread of a 3-launch timing) the lane
-    // mapping that re-reads the fewest bytes from HBM wins: mapping 7, whose
-    // waves of one block stage overlapping neighbourhoods from one CU at the
-    // same time (1.4x the algorithmic bytes on a 4096-sub-grid level, against
-    // 4.3x for mapping 3), then 5, 3 and 0.
-    auto rank = [](int w) { return w / 100 == 7 ? 0 : w / 100 == 5 ? 1 : w / 100 == 3 ? 2 : 3; };
+    // Within 1% of the fastest (the spread of a 3-launch timing) the variant
+    // that re-reads the fewest bytes from HBM wins: mapping 7 over 4 waves,
+    // whose waves of one block stage overlapping neighbourhoods from one CU
+    // at the same time (1.4x the algorithmic bytes on a 4096-sub-grid level,
+    // 1.9x over 2 waves, 4.3x for mapping 3), then the other splits, mapping
+    // 5, 3 and 0.
+    auto rank = [](int w) {
+        return w == 704 ? 0 : w / 100 == 7 ? 1 : w / 100 == 5 ? 2 : w / 100 == 3 ? 3 : 4;
+    };
     float best_near = best;
     for (const auto& t : timed) {
         if (t.second > best * 1.01f) continue;
