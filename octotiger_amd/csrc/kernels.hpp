@@ -492,8 +492,8 @@ struct LevelArgs {
     const Tables* tab;
     const uint64_t* mixbits;   // per coarse offset: byte k, bit j = far pair evaluated in FP32
     // mixed-precision plans, lane mappings 3 and 7: the mixed-precision
-    // records, and per far coarse offset (far_delta order) 1 where every far
-    // pair of the offset is evaluated in FP32
+    // records, and per entry of the far schedule (Tables::sched) 1 where
+    // every far pair of the entry's offset is evaluated in FP32
     const double* recm;
     const uint8_t* mixoff;
     const double* mon;      // [nslots][512] monopoles (p2p window of leaf targets)
