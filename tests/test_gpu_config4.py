@@ -80,3 +80,26 @@ def test_level_config4_full_size(ctx, c4, precision):
         assert st["mixed_far_pairs"] > (0.8 if st["variant"] // 100 == 7 else 0.96) * st["far_pairs"]
         assert 0 < en <= MIXED_COMPONENT_TOL
         assert ok
+
+
+def test_level_config4_drhodt_full_size(ctx, c4):
+    """DRHODT, the first half of the production step (compute_fmm(DRHODT) then
+    compute_fmm(RHO), src/node_server_actions_3.cpp:591-592), at full size:
+    per component normwise <= 1e-12 against the oracle, autotuned."""
+    lv = c4[0]
+    print("config4 DRHODT: computing the oracle", flush=True)
+    Lr, _ = oracle.level_interactions(lv, type_=_abi.DRHODT)
+    dl = hip.LevelOnDevice(lv)
+    stream = hip.Stream()
+    plan = dl.plan(ctx, type_=_abi.DRHODT, flags=_abi.PLAN_USE_GRAPH | _abi.PLAN_AUTOTUNE)
+    st = plan.stats()
+    plan.execute(stream.handle)
+    stream.synchronize()
+    L = dl.L.numpy()
+    plan.close()
+    stream.destroy()
+    dl.free()
+    tg = lv.targets
+    en = normwise(L[tg], Lr[tg])
+    print(f"config4 DRHODT variant {st['variant']}: normwise {en:.3e}", flush=True)
+    assert en <= TOL

@@ -6,8 +6,9 @@ Gate: per level and component, max over cells |gpu - ref| / max |ref| <= 1e-12
 for L, L_c and the leaf phi / g (FP64); multipoles of non-leaf cells within
 1e-12 of the size of their terms (m0 dx^order: moments about the centre of mass
 cancel towards the noise level of the masses, so a relative gate on the result
-would measure that cancellation); the GPU's total force and torque over all
-leaf cells vanish like the oracle's.
+would measure that cancellation; DRHODT sources are signed, and there the
+multipoles are gated normwise per component like L); the GPU's total force and
+torque over all leaf cells vanish like the oracle's.
 """
 import numpy as np
 import pytest
@@ -45,11 +46,16 @@ def compare(tree, got, ref, rho=True, tol=TOL):
     for li, (lv, g, r) in enumerate(zip(tree.levels, got, ref)):
         nl = lv.kind == _abi.SLOT_NONLEAF
         if nl.any():
-            # moments about the centre of mass cancel towards the noise level, so
-            # they are gated against the size of their terms, m0 dx^order
-            order = np.array([0] + [1] * 3 + [2] * 6 + [3] * 10)
-            scale = np.abs(r["M"][nl][..., :1]) * lv.dx ** order
-            worst[f"M{li}"] = float((np.abs(g["M"][nl] - r["M"][nl]) / scale).max())
+            if rho:
+                # moments about the centre of mass cancel towards the noise level,
+                # so they are gated against the size of their terms, m0 dx^order
+                order = np.array([0] + [1] * 3 + [2] * 6 + [3] * 10)
+                scale = np.abs(r["M"][nl][..., :1]) * lv.dx ** order
+                worst[f"M{li}"] = float((np.abs(g["M"][nl] - r["M"][nl]) / scale).max())
+            else:
+                # DRHODT sources are signed, so a cell's m0 can cancel while its
+                # higher moments do not: per component, normwise over the level
+                worst[f"M{li}"] = normwise(g["M"][nl], r["M"][nl])
         worst[f"com{li}"] = np.abs(g["com"] - r["com"]).max() / tree.size
         worst[f"L{li}"] = normwise(g["L"], r["L"])
         if rho:
@@ -133,9 +139,14 @@ def test_tree_without_graph_matches_graph(ctx, name):
     d2.free()
 
 
-def test_small_tree_drhodt_matches_oracle(ctx):
-    t = workloads.small_tree()
-    ref_rho = oracle.tree_solve(t)
+@pytest.mark.parametrize("name", ["small", "c5"])
+def test_tree_drhodt_matches_oracle(ctx, name):
+    """DRHODT solve (monopoles of the leaves as the source, the RHO solve's
+    centres of mass), on the small tree and at full C5 size."""
+    t = workloads.small_tree() if name == "small" else workloads.config5()
+    print(f"tree {t.name} DRHODT", flush=True)
+    nth = 16 if name == "c5" else None
+    ref_rho = oracle.tree_solve(t, nthreads=nth)
     _, dev, _ = run(ctx, t)                      # leaves the RHO centres of mass on the device
     rng = np.random.default_rng(5)
     for lv, d in zip(t.levels, dev.levels):
@@ -143,7 +154,7 @@ def test_small_tree_drhodt_matches_oracle(ctx):
                                       (lv.kind == _abi.SLOT_LEAF)[:, None])
         d["mon"].upload(lv.mon)
     _, _, got = run(ctx, t, dev=dev, type_=_abi.DRHODT)
-    ref = oracle.tree_solve(t, type_=_abi.DRHODT, coms=[o["com"] for o in ref_rho])
+    ref = oracle.tree_solve(t, type_=_abi.DRHODT, coms=[o["com"] for o in ref_rho], nthreads=nth)
     compare(t, got, ref, rho=False)
     dev.free()
 
