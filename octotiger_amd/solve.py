@@ -23,6 +23,9 @@ def main(argv=None):
     ap.add_argument("--tree", choices=("c5", "small", "sphere"), default="c5")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    # a solve of a few ms reaches its steady rate only after some tens of ms
+    # of back-to-back replays: further untimed replays until this many ms
+    ap.add_argument("--warm-ms", type=float, default=150.0)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
@@ -47,6 +50,14 @@ def main(argv=None):
         plan.execute(s.handle)
     s.synchronize()
     e0, e1 = hip.Event(), hip.Event()
+    spent = 0.0
+    while spent < args.warm_ms:
+        e0.record(s)
+        for _ in range(10):
+            plan.execute(s.handle)
+        e1.record(s)
+        e1.synchronize()
+        spent += e0.elapsed_ms(e1)
     e0.record(s)
     for _ in range(args.steps):
         plan.execute(s.handle)
