@@ -570,6 +570,22 @@ __device__ __forceinline__ void form_sgprs(const unsigned (&v)[8]) {
 #endif
 }
 
+// The 8 low bits of e spread to the 8 bytes of a 64-bit lane mask (bit g ->
+// byte g all ones): x * 0x204081 places bit i of a nibble at bit 8i with no
+// carries (the four shifted copies do not overlap), the mask keeps those
+// bits and * 0xff fills their bytes.
+__device__ __forceinline__ uint64_t spread_groups(unsigned e) {
+    const unsigned lo = (((e & 15u) * 0x204081u) & 0x01010101u) * 0xffu;
+    const unsigned hi = ((((e >> 4) & 15u) * 0x204081u) & 0x01010101u) * 0xffu;
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// byte b repeated over the 8 bytes of a 64-bit lane mask
+__device__ __forceinline__ uint64_t repeat_bytes(unsigned b) {
+    const unsigned w = b * 0x01010101u;
+    return ((uint64_t)w << 32) | w;
+}
+
 template <typename T>
 __device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* p) {
     return (const __attribute__((address_space(4))) T*)p;
@@ -728,17 +744,22 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
         issue();
     }
     for (int q = q0; q < nfd; q += SPLIT) {
-        // the lane's records of the image: those of its class from each
-        // partner, unless that partner's slot is empty; kind 2 classes on the
-        // far side of the axis read records 4..7 as their records 0..3
+        // the lanes that read record j of the image: those of the classes in
+        // byte j of the entry's step masks, unless their group's partner slot
+        // is empty; kind 2 classes on the far side of the axis read records
+        // 4..7 as their records 0..3.  The masks are formed in scalar
+        // registers, one 64-bit lane mask per step (lane = group * 8 + class).
+        const uint64_t ra = tab->sched_rowa[q], rb = tab->sched_rowb[q];
+        // (emptyb_next is formed between the branches of issue(), after which
+        // the compiler no longer knows it to be uniform)
+        const uint64_t ea = spread_groups(empty_next);
+        const uint64_t eb = spread_groups(__builtin_amdgcn_readfirstlane(emptyb_next));
+        auto step_mask = [&](int j) -> uint64_t {
+            return (repeat_bytes((unsigned)(ra >> (8 * j)) & 0xffu) & ~ea) |
+                (repeat_bytes((unsigned)(rb >> (8 * j)) & 0xffu) & ~eb);
+        };
         const int lq = lane();
         const unsigned k = (unsigned)lq & 7u, c = (unsigned)lq >> 3;
-        const uint64_t ma = tab->sched_fara[q], mb = tab->sched_farb[q];
-        const unsigned wa = k < 4 ? (unsigned)ma : (unsigned)(ma >> 32);
-        const unsigned wb = k < 4 ? (unsigned)mb : (unsigned)(mb >> 32);
-        const unsigned fa = ((empty_next >> c) & 1u) ? 0u : wa;
-        const unsigned fbb = ((emptyb_next >> c) & 1u) ? 0u : wb;
-        const unsigned fb = __builtin_amdgcn_ubfe(fa | fbb, 8 * (k & 3u), 8);
         const double2* recs = stage + c * kDirGroup + ((tab->sched_half[q] >> k) & 1u) * 48;
         const bool f32 = MIXED && mixoff[q];   // uniform
         if (q + SPLIT < nfd) bases(q + SPLIT, source(q + SPLIT));
@@ -750,14 +771,15 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             for (int i = 0; i < 20; ++i) Lf[i] = 0.0f;
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                if (fb & (1u << j)) m2l_far_m32<RHO>(recs + j * 12, X0, X1, X2, inv_mc0, mc, Lf, C);
+                if (__builtin_amdgcn_inverse_ballot_w64(step_mask(j)))
+                    m2l_far_m32<RHO>(recs + j * 12, X0, X1, X2, inv_mc0, mc, Lf, C);
             // the offset's FP32 partial sums join the FP64 expansion
 #pragma unroll
             for (int i = 0; i < 20; ++i) L[i] += (double)Lf[i];
         } else {
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                if (fb & (1u << j)) {
+                if (__builtin_amdgcn_inverse_ballot_w64(step_mask(j))) {
                     Rec cur;
 #pragma unroll
                     for (int i = 0; i < 12; ++i) cur.v[i] = recs[j * 12 + i];

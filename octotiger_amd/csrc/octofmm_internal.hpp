@@ -61,10 +61,15 @@ struct Tables {
     // the image's records 0..7 taken from offset a's partner (sched_fara)
     // and from b's (sched_farb), and bit k of sched_half set where class k
     // reads records 4..7 as its records 0..3 (kind 2, k_axis = 1).
+    // sched_rowa / sched_rowb hold the same bits transposed: byte j, bit k
+    // set where class k's step j reads a record of a's (b's) partner, so a
+    // wave forms the lane mask of step j from scalar registers.
     int32_t nsched;
     int32_t sched[kMaxDelta];
     uint64_t sched_fara[kMaxDelta];
     uint64_t sched_farb[kMaxDelta];
+    uint64_t sched_rowa[kMaxDelta];
+    uint64_t sched_rowb[kMaxDelta];
     uint32_t sched_half[kMaxDelta];
     double four[kStencilBox][4];      // p2p constants (-1/r, x/r^3, y/r^3, z/r^3) by fine offset
     uint8_t root_excluded[kStencilBox];   // root kernel: near offsets and self

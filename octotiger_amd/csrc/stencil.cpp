@@ -102,9 +102,17 @@ void build_far_schedule(Tables& t) {
             fa |= ma << (8 * k);
             fb |= mb << (8 * k);
         }
+        uint64_t ra = 0, rb = 0;
+        for (int k = 0; k < 8; ++k)
+            for (int j = 0; j < 8; ++j) {
+                ra |= ((fa >> (8 * k + j)) & 1) << (8 * j + k);
+                rb |= ((fb >> (8 * k + j)) & 1) << (8 * j + k);
+            }
         t.sched[q] = e;
         t.sched_fara[q] = fa;
         t.sched_farb[q] = fb;
+        t.sched_rowa[q] = ra;
+        t.sched_rowb[q] = rb;
         t.sched_half[q] = half;
     }
     t.nsched = n;

@@ -40,6 +40,9 @@ int main() {
                 for (int s = 0; s < 8; ++s) {
                     if ((t.sched_fara[q] >> (8 * k + s)) & 1) if (rd[s + off] != a) ++fails;
                     if ((t.sched_farb[q] >> (8 * k + s)) & 1) if (rd[s + off] != b || kind == 0) ++fails;
+                    // the step masks the kernels read: the same bits by step, then class
+                    if (((t.sched_rowa[q] >> (8 * s + k)) & 1) != ((t.sched_fara[q] >> (8 * k + s)) & 1)) ++fails;
+                    if (((t.sched_rowb[q] >> (8 * s + k)) & 1) != ((t.sched_farb[q] >> (8 * k + s)) & 1)) ++fails;
                 }
             }
             steps += maxs;
