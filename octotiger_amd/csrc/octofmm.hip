@@ -1048,15 +1048,18 @@ __device__ void p2m_item(const LevelArgs& a, int slot, int ordinal, int w, const
         issue();
     }
     for (int q = q0; q < nlive; q += SPLIT) {
+        // lanes of step j: the classes in byte j of the offset's pair rows,
+        // in the groups whose partner is in a refined slot (lane mask formed
+        // in scalar registers, lane = group * 8 + class)
         const int di = s_live[q];
-        const unsigned all = (unsigned)((tab->farbits[di] | tab->nearbits[di]) >> (8 * (l & 7))) & 0xffu;
-        const unsigned pb = ((absent_next >> c) & 1u) ? 0u : all;
+        const uint64_t rows = tab->allrows[di];
+        const uint64_t gone = spread_groups(absent_next);
         if (q + SPLIT < nlive) bases(s_live[q + SPLIT]);
         __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): this wave's image is in LDS
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            if (pb & (1u << j)) {
+            if (__builtin_amdgcn_inverse_ballot_w64(repeat_bytes((unsigned)(rows >> (8 * j)) & 0xffu) & ~gone)) {
                 Rec cur;
 #pragma unroll
                 for (int i = 0; i < 12; ++i) cur.v[i] = group[j * 12 + i];

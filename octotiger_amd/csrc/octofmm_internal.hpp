@@ -39,6 +39,7 @@ struct Tables {
     int8_t delta[kMaxDelta][4];       // coarse offsets (x, y, z, 0)
     uint64_t farbits[kMaxDelta];      // byte k: bit j set if (delta, j, k) is far
     uint64_t nearbits[kMaxDelta];     // byte k: bit j set if (delta, j, k) is near (self excluded)
+    uint64_t allrows[kMaxDelta];      // farbits | nearbits transposed: byte j, bit k
     int32_t nfar_delta;               // coarse offsets with a far pair for some child class,
     int32_t far_delta[kMaxDelta];     // in delta order (delta 0 has none: it is all near / self)
     // Far schedule of the direct-staging kernels: one staged image of 8

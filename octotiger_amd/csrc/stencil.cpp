@@ -33,6 +33,15 @@ inline int box_index(int x, int y, int z) {
     return (x + 5) * kStencilInx * kStencilInx + (y + 5) * kStencilInx + (z + 5);
 }
 
+// An 8 x 8 bit matrix held as 8 bytes, transposed: bit j of byte k becomes
+// bit k of byte j.
+uint64_t transpose_bytes(uint64_t m) {
+    uint64_t r = 0;
+    for (int k = 0; k < 8; ++k)
+        for (int j = 0; j < 8; ++j) r |= ((m >> (8 * k + j)) & 1) << (8 * j + k);
+    return r;
+}
+
 int find_delta(const Tables& t, int x, int y, int z) {
     for (int i = 0; i < t.ndelta; ++i)
         if (t.delta[i][0] == x && t.delta[i][1] == y && t.delta[i][2] == z) return i;
@@ -102,17 +111,11 @@ void build_far_schedule(Tables& t) {
             fa |= ma << (8 * k);
             fb |= mb << (8 * k);
         }
-        uint64_t ra = 0, rb = 0;
-        for (int k = 0; k < 8; ++k)
-            for (int j = 0; j < 8; ++j) {
-                ra |= ((fa >> (8 * k + j)) & 1) << (8 * j + k);
-                rb |= ((fb >> (8 * k + j)) & 1) << (8 * j + k);
-            }
         t.sched[q] = e;
         t.sched_fara[q] = fa;
         t.sched_farb[q] = fb;
-        t.sched_rowa[q] = ra;
-        t.sched_rowb[q] = rb;
+        t.sched_rowa[q] = transpose_bytes(fa);
+        t.sched_rowb[q] = transpose_bytes(fb);
         t.sched_half[q] = half;
     }
     t.nsched = n;
@@ -217,8 +220,10 @@ bool build_tables(double theta, Tables& t, char* err, size_t errlen) {
     t.ndelta = n;
     t.max_delta = maxd;
     t.nfar_delta = 0;
-    for (int i = 0; i < n; ++i)
+    for (int i = 0; i < n; ++i) {
         if (t.farbits[i]) t.far_delta[t.nfar_delta++] = i;
+        t.allrows[i] = transpose_bytes(t.farbits[i] | t.nearbits[i]);
+    }
     build_far_schedule(t);
     int nf = 0, nn = 0;
     for (int i = 0; i < n; ++i) {
