@@ -586,6 +586,17 @@ __device__ __forceinline__ uint64_t repeat_bytes(unsigned b) {
     return ((uint64_t)w << 32) | w;
 }
 
+// A uniform address formed in scalar registers at this point: a load that
+// adds a 32-bit lane offset to it then takes the scalar-base form, with no
+// per-lane 64-bit address arithmetic (the constraint exists only in the
+// device pass).
+__device__ __forceinline__ const char* uniform_ptr(const char* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(p));
+#endif
+    return p;
+}
+
 template <typename T>
 __device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* p) {
     return (const __attribute__((address_space(4))) T*)p;
@@ -693,12 +704,12 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             const unsigned loff = (unsigned)lo * 16u;
 #pragma unroll
             for (int gi = 0; gi < 8; ++gi)
-                __builtin_amdgcn_global_load_lds(rb + (size_t)nrec[gi] * kRecBytes + loff, (lds_ptr)(st + gi * kDirGroup),
-                                                 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(uniform_ptr(rb + (size_t)nrec[gi] * kRecBytes) + loff,
+                                                 (lds_ptr)(st + gi * kDirGroup), 16, 0, 0);
             if (lo < 32) {
 #pragma unroll
                 for (int gi = 0; gi < 8; ++gi)
-                    __builtin_amdgcn_global_load_lds(rb + (size_t)nrec[gi] * kRecBytes + 1024 + loff,
+                    __builtin_amdgcn_global_load_lds(uniform_ptr(rb + (size_t)nrec[gi] * kRecBytes + 1024) + loff,
                                                      (lds_ptr)(st + gi * kDirGroup + 64), 16, 0, 0);
             }
         } else {
@@ -713,8 +724,8 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             if (lo < 48) {
 #pragma unroll
                 for (int gi = 0; gi < 8; ++gi)
-                    __builtin_amdgcn_global_load_lds(rb + (size_t)nrec[gi] * kRecBytes + offa, (lds_ptr)(st + gi * kDirGroup),
-                                                     16, 0, 0);
+                    __builtin_amdgcn_global_load_lds(uniform_ptr(rb + (size_t)nrec[gi] * kRecBytes) + offa,
+                                                     (lds_ptr)(st + gi * kDirGroup), 16, 0, 0);
             }
             const int di = entry_next & 0xff;
             const int ax = tab->delta[di][0], ay = tab->delta[di][1], az = tab->delta[di][2];
@@ -727,7 +738,7 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             if (lo < 48) {
 #pragma unroll
                 for (int gi = 0; gi < 8; ++gi)
-                    __builtin_amdgcn_global_load_lds(rb + (size_t)brec[gi] * kRecBytes + offb,
+                    __builtin_amdgcn_global_load_lds(uniform_ptr(rb + (size_t)brec[gi] * kRecBytes) + offb,
                                                      (lds_ptr)(st + gi * kDirGroup + 48), 16, 0, 0);
             }
         }
