@@ -754,6 +754,8 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
         bases(q0, source(q0));
         issue();
     }
+    // the lane's group of 8 records in the image
+    const double2* group = stage + (lane() >> 3) * kDirGroup;
     for (int q = q0; q < nfd; q += SPLIT) {
         // the lanes that read record j of the image: those of the classes in
         // byte j of the entry's step masks, unless their group's partner slot
@@ -769,9 +771,8 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
             return (repeat_bytes((unsigned)(ra >> (8 * j)) & 0xffu) & ~ea) |
                 (repeat_bytes((unsigned)(rb >> (8 * j)) & 0xffu) & ~eb);
         };
-        const int lq = lane();
-        const unsigned k = (unsigned)lq & 7u, c = (unsigned)lq >> 3;
-        const double2* recs = stage + c * kDirGroup + ((tab->sched_half[q] >> k) & 1u) * 48;
+        const double2* recs =
+            __builtin_amdgcn_inverse_ballot_w64(repeat_bytes(tab->sched_half[q] & 0xffu)) ? group + 48 : group;
         const bool f32 = MIXED && mixoff[q];   // uniform
         if (q + SPLIT < nfd) bases(q + SPLIT, source(q + SPLIT));
         __builtin_amdgcn_s_waitcnt(0x3F70);   // vmcnt(0): this wave's image is in LDS
