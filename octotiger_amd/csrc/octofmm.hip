@@ -2542,32 +2542,43 @@ static octofmm_status plan_autotune_run(octofmm_plan* p) {
     hipEvent_t e0, e1;
     HIPC(c, hipEventCreate(&e0));
     HIPC(c, hipEventCreate(&e1));
-    float best = 1e30f;
-    int best_w = 8;
+    // Two rounds, the variants in list order and then in reverse, each
+    // variant's times summed: the clock of a GPU that has just come out of
+    // idle still rises over the first tens of milliseconds, which in one
+    // round in list order favours the variants timed last.
     std::vector<std::pair<int, float>> timed;
-    for (int w : kTunedVariants) {
-        if (!variant_allowed(p, w)) continue;
-        p->variant = w;
-        s = plan_launch(p, st, 2);   // warm-up
-        if (s != OCTOFMM_OK) return s;
-        HIPC(c, hipEventRecord(e0, st));
-        for (int i = 0; i < 3; ++i) {
-            s = plan_launch(p, st, 2);
+    for (int w : kTunedVariants)
+        if (variant_allowed(p, w)) timed.emplace_back(w, 0.f);
+    const int nv = (int)timed.size();
+    for (int round = 0; round < 2; ++round) {
+        for (int i = 0; i < nv; ++i) {
+            auto& t = timed[round == 0 ? i : nv - 1 - i];
+            p->variant = t.first;
+            s = plan_launch(p, st, 2);   // warm-up
             if (s != OCTOFMM_OK) return s;
-        }
-        HIPC(c, hipEventRecord(e1, st));
-        HIPC(c, hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPC(c, hipEventElapsedTime(&ms, e0, e1));
-        timed.emplace_back(w, ms);
-        if (ms < best) {
-            best = ms;
-            best_w = w;
+            HIPC(c, hipEventRecord(e0, st));
+            for (int l = 0; l < 3; ++l) {
+                s = plan_launch(p, st, 2);
+                if (s != OCTOFMM_OK) return s;
+            }
+            HIPC(c, hipEventRecord(e1, st));
+            HIPC(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPC(c, hipEventElapsedTime(&ms, e0, e1));
+            t.second += ms;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // Within 1% of the fastest (the spread of a 3-launch timing) the variant
+    float best = 1e30f;
+    int best_w = 8;
+    for (const auto& t : timed) {
+        if (t.second < best) {
+            best = t.second;
+            best_w = t.first;
+        }
+    }
+    // Within 1% of the fastest (the spread of these timings) the variant
     // that re-reads the fewest bytes from HBM wins: mapping 7 over 4 waves,
     // whose waves of one block stage overlapping neighbourhoods from one CU
     // at the same time (1.4x the algorithmic bytes on a 4096-sub-grid level,
