@@ -570,22 +570,6 @@ __device__ __forceinline__ void form_sgprs(const unsigned (&v)[8]) {
 #endif
 }
 
-// The 8 low bits of e spread to the 8 bytes of a 64-bit lane mask (bit g ->
-// byte g all ones): x * 0x204081 places bit i of a nibble at bit 8i with no
-// carries (the four shifted copies do not overlap), the mask keeps those
-// bits and * 0xff fills their bytes.
-__device__ __forceinline__ uint64_t spread_groups(unsigned e) {
-    const unsigned lo = (((e & 15u) * 0x204081u) & 0x01010101u) * 0xffu;
-    const unsigned hi = ((((e >> 4) & 15u) * 0x204081u) & 0x01010101u) * 0xffu;
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// byte b repeated over the 8 bytes of a 64-bit lane mask
-__device__ __forceinline__ uint64_t repeat_bytes(unsigned b) {
-    const unsigned w = b * 0x01010101u;
-    return ((uint64_t)w << 32) | w;
-}
-
 // A uniform address formed in scalar registers at this point: a load that
 // adds a 32-bit lane offset to it then takes the scalar-base form, with no
 // per-lane 64-bit address arithmetic (the constraint exists only in the
@@ -767,10 +751,7 @@ __device__ void m2l_item_direct(const LevelArgs& a, int slot, int flags, const i
         // the compiler no longer knows it to be uniform)
         const uint64_t ea = spread_groups(empty_next);
         const uint64_t eb = spread_groups(__builtin_amdgcn_readfirstlane(emptyb_next));
-        auto step_mask = [&](int j) -> uint64_t {
-            return (repeat_bytes((unsigned)(ra >> (8 * j)) & 0xffu) & ~ea) |
-                (repeat_bytes((unsigned)(rb >> (8 * j)) & 0xffu) & ~eb);
-        };
+        auto step_mask = [&](int j) -> uint64_t { return step_lanes(ra, rb, ea, eb, j); };
         const double2* recs =
             __builtin_amdgcn_inverse_ballot_w64(repeat_bytes(tab->sched_half[q] & 0xffu)) ? group + 48 : group;
         const bool f32 = MIXED && mixoff[q];   // uniform
@@ -1071,7 +1052,7 @@ __device__ void p2m_item(const LevelArgs& a, int slot, int ordinal, int w, const
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            if (__builtin_amdgcn_inverse_ballot_w64(repeat_bytes((unsigned)(rows >> (8 * j)) & 0xffu) & ~gone)) {
+            if (__builtin_amdgcn_inverse_ballot_w64(step_lanes(rows, 0, gone, 0, j))) {
                 Rec cur;
 #pragma unroll
                 for (int i = 0; i < 12; ++i) cur.v[i] = group[j * 12 + i];

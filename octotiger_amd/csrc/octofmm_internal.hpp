@@ -135,6 +135,34 @@ constexpr int kTunedVariants[9] = {8, 4, 2, 304, 302, 504, 704, 702, 708};
 // stencil fails the reference's symmetry / width checks.
 bool build_tables(double theta, Tables& t, char* err, size_t errlen);
 
+// Lane masks of the direct-staging kernels, whose lane is group * 8 + class
+// (group = the partner coarse cell's slot in the wave's block).
+//
+// The 8 low bits of e spread to the 8 bytes of a 64-bit lane mask (bit g ->
+// byte g all ones): x * 0x204081 places bit i of a nibble at bit 8i with no
+// carries (the four shifted copies do not overlap), the mask keeps those
+// bits and * 0xff fills their bytes.
+__host__ __device__ inline uint64_t spread_groups(unsigned e) {
+    const unsigned lo = (((e & 15u) * 0x204081u) & 0x01010101u) * 0xffu;
+    const unsigned hi = ((((e >> 4) & 15u) * 0x204081u) & 0x01010101u) * 0xffu;
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// byte b repeated over the 8 bytes of a 64-bit lane mask
+__host__ __device__ inline uint64_t repeat_bytes(unsigned b) {
+    const unsigned w = b * 0x01010101u;
+    return ((uint64_t)w << 32) | w;
+}
+
+// The lanes that evaluate step j of an image: the classes in byte j of the
+// step rows of the records taken from offset a's partners (rowa) and from
+// b's (rowb), in every group except those whose partner is gone (gone_a,
+// gone_b: spread_groups of the groups' bits).
+__host__ __device__ inline uint64_t step_lanes(uint64_t rowa, uint64_t rowb, uint64_t gone_a, uint64_t gone_b, int j) {
+    return (repeat_bytes((unsigned)(rowa >> (8 * j)) & 0xffu) & ~gone_a) |
+        (repeat_bytes((unsigned)(rowb >> (8 * j)) & 0xffu) & ~gone_b);
+}
+
 __host__ __device__ inline int child_of(int x, int y, int z) { return ((x & 1) << 2) | ((y & 1) << 1) | (z & 1); }
 __host__ __device__ inline int coarse_of(int x, int y, int z) { return ((x >> 1) << 4) | ((y >> 1) << 2) | (z >> 1); }
 // record index of cell (x,y,z) inside its slot: coarse-major, child-minor

@@ -4,6 +4,8 @@
 // class's reads from sched_fara / sched_farb / sched_half -- and counts how
 // often each far (delta, k, j) pair of the tables is evaluated: exactly once
 // at every theta, with 196 lockstep steps per cell instead of 208 at 0.5.
+// Also checks the lane masks the kernels form from the transposed step rows
+// (spread_groups, repeat_bytes, step_lanes) against the per-lane rule.
 #include "octofmm_internal.hpp"
 
 #include <cstdio>
@@ -13,6 +15,16 @@ using namespace octofmm;
 
 int main() {
     int fails = 0;
+    // the lane-mask helpers, exhaustively: byte g of spread_groups(e) is all
+    // ones exactly where bit g of e is set, and every byte of repeat_bytes(b) is b
+    for (unsigned e = 0; e < 256; ++e) {
+        const uint64_t sg = spread_groups(e), rb = repeat_bytes(e);
+        for (int g = 0; g < 8; ++g) {
+            if (((sg >> (8 * g)) & 0xff) != (((e >> g) & 1) ? 0xffu : 0u)) ++fails;
+            if (((rb >> (8 * g)) & 0xff) != e) ++fails;
+        }
+    }
+    if (fails) printf("lane-mask helpers wrong: %d\n", fails);
     for (double th : {0.5, 0.49, 0.45, 0.4, 0.34}) {
         static Tables t; char err[256];
         if (!build_tables(th, t, err, sizeof err)) { printf("fail %s\n", err); return 1; }
@@ -44,9 +56,29 @@ int main() {
                     if (((t.sched_rowa[q] >> (8 * s + k)) & 1) != ((t.sched_fara[q] >> (8 * k + s)) & 1)) ++fails;
                     if (((t.sched_rowb[q] >> (8 * s + k)) & 1) != ((t.sched_farb[q] >> (8 * k + s)) & 1)) ++fails;
                 }
+                // the lane mask of each step (step_lanes) against the per-lane
+                // rule: lane (group g, class k) evaluates step s when class k
+                // reads record s from a and g's partner at a is there, or from b
+                // and g's partner at b is there; over a sample of empty-group sets
+                for (unsigned ea = 0; ea < 256; ea += 37)
+                    for (unsigned eb = 0; eb < 256; eb += 53)
+                        for (int s = 0; s < 8; ++s) {
+                            const uint64_t m = step_lanes(t.sched_rowa[q], t.sched_rowb[q], spread_groups(ea),
+                                                          spread_groups(eb), s);
+                            for (int g = 0; g < 8; ++g) {
+                                const bool fa = ((t.sched_fara[q] >> (8 * k + s)) & 1) && !((ea >> g) & 1);
+                                const bool fb = ((t.sched_farb[q] >> (8 * k + s)) & 1) && !((eb >> g) & 1);
+                                if ((((m >> (8 * g + k)) & 1) != 0) != (fa || fb)) ++fails;
+                            }
+                        }
             }
             steps += maxs;
         }
+        // the p2m kernel's step rows: far or near bits transposed
+        for (int d = 0; d < t.ndelta; ++d)
+            for (int k = 0; k < 8; ++k)
+                for (int j = 0; j < 8; ++j)
+                    if (((t.allrows[d] >> (8 * j + k)) & 1) != (((t.farbits[d] | t.nearbits[d]) >> (8 * k + j)) & 1)) ++fails;
         int bad = 0, nfar = 0;
         for (int d = 0; d < t.ndelta; ++d) for (int bit = 0; bit < 64; ++bit) {
             const int f = (t.farbits[d] >> bit) & 1; nfar += f;
