@@ -60,8 +60,9 @@ int main() {
                 // rule: lane (group g, class k) evaluates step s when class k
                 // reads record s from a and g's partner at a is there, or from b
                 // and g's partner at b is there; over a sample of empty-group sets
-                for (unsigned ea = 0; ea < 256; ea += 37)
-                    for (unsigned eb = 0; eb < 256; eb += 53)
+                static const unsigned kEmpty[] = {0x00, 0xff, 0x01, 0x80, 0x55, 0xaa, 0x0f, 0xf0, 0x25, 0xc8};
+                for (unsigned ea : kEmpty)
+                    for (unsigned eb : kEmpty)
                         for (int s = 0; s < 8; ++s) {
                             const uint64_t m = step_lanes(t.sched_rowa[q], t.sched_rowb[q], spread_groups(ea),
                                                           spread_groups(eb), s);
