@@ -165,8 +165,15 @@ __device__ __forceinline__ void dipole_rows(const double* m, const double* D, do
 // a[k] = factor-scaled octupole difference (M2L) or partner octupole (p2m).
 __device__ __forceinline__ void am_correction_t(double d2, double d3x00, double d3x11, double d3x22, double d3x01,
     double d3x02, double d3x12, const double* a, double* C) {
+    // 6.0 in a scalar register pair: as a literal it makes the compiler use
+    // the two-operand FMA, whose tied destination costs a register copy for
+    // each of U0 and U10 (the three-operand form takes no 64-bit literal)
+    double six = 6.0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(six));
+#endif
     const double t2 = 3.0 * d2;
-    const double U0 = fma(6.0, d3x00, t2);
+    const double U0 = fma(six, d3x00, t2);
     const double U1 = 3.0 * d3x01;
     const double U2 = 3.0 * d3x02;
     const double U3 = d2 + d3x11 + d3x00;
@@ -174,10 +181,10 @@ __device__ __forceinline__ void am_correction_t(double d2, double d3x00, double 
     const double U5 = d2 + d3x22 + d3x00;
     const double U7 = d3x02;
     const double U8 = d3x01;
-    const double U10 = fma(6.0, d3x11, t2);
+    const double U10 = fma(six, d3x11, t2);
     const double U12 = d2 + d3x22 + d3x11;
     const double U13 = 3.0 * d3x12;
-    const double U14 = fma(6.0, d3x22, t2);
+    const double U14 = fma(six, d3x22, t2);
     double c0 = C[0], c1 = C[1], c2 = C[2];
     c0 = fma(-a[0], U0, c0);  c1 = fma(-a[0], U1, c1);  c2 = fma(-a[0], U2, c2);
     c0 = fma(-a[1], U1, c0);  c1 = fma(-a[1], U3, c1);  c2 = fma(-a[1], U4, c2);
